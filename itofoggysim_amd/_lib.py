@@ -1,0 +1,102 @@
+"""ctypes binding of libifs.so (the HIP library built from csrc/).
+
+The library must be built in-tree (``make -C itofoggysim_amd``).  There is no
+CPU fallback: if the library is missing every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libifs.so")
+
+IFS_OK = 0
+ERRORS = {
+    -1: "IFS_E_ARG (invalid argument)",
+    -2: "IFS_E_NOMEM (device allocation failed)",
+    -3: "IFS_E_HIP (HIP runtime error)",
+    -4: "IFS_E_NODEVICE (no GPU visible)",
+    -5: "IFS_E_NOCONV (not converged)",
+    -6: "IFS_E_BREAKDOWN (solver breakdown)",
+}
+
+
+class IfsError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where}: {ERRORS.get(code, code)}")
+        self.code = code
+
+
+class Cluster(ctypes.Structure):
+    _fields_ = [("n_spheres", ctypes.c_int32), ("centres", ctypes.c_void_p), ("radii", ctypes.c_void_p),
+                ("m_rel", ctypes.c_void_p), ("lmax", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_double), ("n_global", ctypes.c_int32), ("z_selector", ctypes.c_int32),
+                ("ref_row_zeroing", ctypes.c_int32), ("use_fmm", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
+                ("fmm_d0", ctypes.c_double), ("fmm_lmax_cap", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("restart", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
+                ("use_x0", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_residual", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+
+# name -> (restype, argtypes); every symbol include/ifs.h declares
+SIGNATURES = {
+    "ifs_create": (_i32, [ctypes.POINTER(_vp), ctypes.POINTER(Cluster), ctypes.POINTER(Params)]),
+    "ifs_destroy": (_i32, [_vp]),
+    "ifs_vector_length": (_i64, [_vp]),
+    "ifs_matvec": (_i32, [_vp, _vp, _vp, _i64]),
+    "ifs_matvec_device": (_i32, [_vp, _vp, _vp, _i64]),
+    "ifs_set_stream": (_i32, [_vp, _vp]),
+    "ifs_get_stream": (_vp, [_vp]),
+    "ifs_get_mie": (_i32, [_vp, _vp, _vp]),
+    "ifs_plane_wave_rhs": (_i32, [_vp, _vp, _i64]),
+    "ifs_solve": (_i32, [_vp, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "ifs_solve_device": (_i32, [_vp, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "ifs_tree_export": (_i32, [_vp, _vp, _vp]),
+    "ifs_tree_scale": (_i32, [_vp, _i32, _vp, _vp]),
+    "ifs_tree_universal_index": (_i32, [_vp, _i32, _i32, _vp]),
+    "ifs_tree_neighbours": (_i32, [_i64, _i64, _i32, _vp]),
+    "ifs_tree_centre": (_i32, [_i64, _i32, _vp]),
+    "ifs_tree_domain_i4": (_i32, [_i64, _i32, _vp, _i32]),
+    "ifs_tree_level_max": (_i32, [_vp, _i64, _i64]),
+    "ifs_fmm_stats": (_i32, [_vp, _vp, _vp]),
+    "ifs_profile_enable": (_i32, [_vp, _i32]),
+    "ifs_profile_read": (_i32, [_vp, _vp, _vp, _vp, _i32]),
+    "ifs_matvec_cost": (_i32, [_vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libifs.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C itofoggysim_amd` "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, where):
+    if rc != IFS_OK:
+        raise IfsError(rc, where)
+    return rc

@@ -1,0 +1,628 @@
+// Host orchestration and the C ABI (include/ifs.h).
+//
+// A context owns: the sphere data on the device, the Mie T-matrix, the
+// translation plan (tasks grouped by target, split into workspace chunks), the
+// workspace, and one HIP stream.  ifs_matvec_device runs
+//     pack (Xu basis -> normalised compact) -> translations -> finalize (T-apply + identity)
+// which replaces lib_mie_ms_solver_calculate_vector_b
+// (FoggySim/src/lib/mie_theory/lib_mie_ms_solver_interface.f90:739-998).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ifs.h"
+#include "ifs_fmm.h"
+#include "ifs_internal.h"
+
+using namespace ifs;
+
+namespace {
+
+#define HIP_OK(expr)                          \
+    do {                                      \
+        if ((expr) != hipSuccess) return IFS_E_HIP; \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t b) {
+        if (p && bytes >= b) return IFS_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (b == 0) return IFS_OK;
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return IFS_E_NOMEM;
+        }
+        bytes = b;
+        return IFS_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// Chunk of a translation batch that fits the workspace.
+struct Chunk {
+    int t_begin = 0, t_end = 0;      // task range (device task array)
+    int s_begin = 0, s_end = 0;      // segment range
+    int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;
+    int maxpairs = 0, maxK = 0, max_nmax = 0, max_Ls = 0;
+    int64_t max_rows_y = 0, max_rows_z = 0;
+    int rowback_blocks = 1;
+};
+
+// A translation batch: tasks from one source pool into one target pool.
+struct Batch {
+    std::vector<Task> tasks;     // sorted by target
+    std::vector<Segment> segs;
+    std::vector<Chunk> chunks;
+    DevBuf d_tasks, d_segs;
+    bool cached = false;         // tables generated once and kept
+    bool tables_ready = false;
+    std::string name;
+};
+
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, int64_t>> totals;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void flush() {
+        for (auto &r : recs) {
+            (void)hipEventSynchronize(r.b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            auto &t = totals[r.name];
+            t.first += ms;
+            t.second += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    ~Profiler() {
+        for (auto &r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+struct ifs_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    ifs_params prm{};
+    int ns = 0;
+    int N = 0;
+    std::vector<double> pos;        // 3*ns
+    std::vector<double> radius;
+    std::vector<std::complex<double>> mrel;
+    std::vector<int32_t> Lj;        // requested per-sphere truncation
+    std::vector<int32_t> Leff;      // min(L_j, N)
+    std::vector<int32_t> active;    // row activity (reference row zeroing)
+    std::vector<int64_t> off;       // compact normalised offsets
+    int64_t ncompact = 0;           // complex entries of the compact vector
+    int64_t nflat = 0;              // complex entries of the flat vector
+
+    DevBuf d_Leff, d_active, d_off, d_snm, d_an, d_bn, d_x, d_b;
+    DevBuf d_xh, d_acc;             // compact normalised input / accumulator
+    DevBuf ws_d, ws_s, ws_ax, ws_y, ws_z;
+    size_t ws_budget = 0;
+
+    Batch direct;                   // sphere -> sphere (direct all pairs)
+    fmm::Fmm *fmm = nullptr;
+
+    // Krylov workspace
+    DevBuf k_V, k_w, k_r, k_part, k_h, k_xtmp;
+    int k_restart = 0;
+
+    Profiler prof;
+    double flops_mv = 0.0, bytes_mv = 0.0;
+};
+
+namespace {
+
+int64_t fidx(int n, int m) { return (int64_t)n * n - 1 + m + n; }
+
+// s(n,m) = sqrt(n(n+1)(2n+1) (n-m)!/(n+m)!) in long double
+std::vector<double> make_snm(int N) {
+    std::vector<double> s((size_t)N * (N + 2));
+    for (int n = 1; n <= N; ++n)
+        for (int m = -n; m <= n; ++m) {
+            long double r = 1.0L;  // (n-m)!/(n+m)!
+            if (m >= 0)
+                for (int i = n - m + 1; i <= n + m; ++i) r /= (long double)i;
+            else
+                for (int i = n + m + 1; i <= n - m; ++i) r *= (long double)i;
+            s[fidx(n, m)] = (double)sqrtl((long double)n * (n + 1) * (2 * n + 1) * r);
+        }
+    return s;
+}
+
+Task make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_t src, int64_t dst) {
+    Task T{};
+    double r = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    T.src = src;
+    T.dst = dst;
+    T.t = k * r;
+    T.cth = r > 0.0 ? std::max(-1.0, std::min(1.0, dv[2] / r)) : 1.0;
+    T.phi = (dv[0] == 0.0 && dv[1] == 0.0) ? 0.0 : std::atan2(dv[1], dv[0]);
+    T.Ls = Ls;
+    T.Lt = Lt;
+    T.K = std::min(Ls, Lt);
+    T.kind = kind;
+    return T;
+}
+
+}  // namespace
+
+namespace ifs {
+
+// Split a target-sorted task list into segments and workspace chunks and upload it.
+int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) {
+    B.segs.clear();
+    B.chunks.clear();
+    const size_t n = B.tasks.size();
+    // per-task sizes
+    auto sizes = [](const Task &T, int64_t &nd, int64_t &nsct, int64_t &nax, int64_t &ny, int64_t &nz) {
+        int L = std::max(T.Ls, T.Lt);
+        nd = wig_size(L, T.K);
+        nsct = sect_size(T.K, T.Lt + T.Ls + 2);
+        nax = ax_size(T.Lt, T.Ls, T.K);
+        ny = 2 * rows_size(T.Ls, T.K);
+        nz = 2 * rows_size(T.Lt, T.K);
+    };
+    int64_t tot_bytes = 0;
+    for (auto &T : B.tasks) {
+        int64_t nd, nsct, nax, ny, nz;
+        sizes(T, nd, nsct, nax, ny, nz);
+        tot_bytes += nd * 8 + (nsct + nax + ny + nz) * 16;
+    }
+    B.cached = allow_cache && (size_t)tot_bytes <= budget_bytes;
+    size_t chunk_budget = B.cached ? (size_t)tot_bytes + 1 : budget_bytes;
+
+    Chunk cur;
+    cur.t_begin = 0;
+    cur.s_begin = 0;
+    int64_t used = 0;
+    auto close_chunk = [&](int t_end) {
+        cur.t_end = t_end;
+        cur.s_end = (int)B.segs.size();
+        B.chunks.push_back(cur);
+        cur = Chunk();
+        cur.t_begin = t_end;
+        cur.s_begin = (int)B.segs.size();
+        used = 0;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        Task &T = B.tasks[i];
+        int64_t nd, nsct, nax, ny, nz;
+        sizes(T, nd, nsct, nax, ny, nz);
+        int64_t bytes = nd * 8 + (nsct + nax + ny + nz) * 16;
+        if (i > (size_t)cur.t_begin && used + bytes > (int64_t)chunk_budget) {
+            // close the open segment at the chunk boundary
+            close_chunk((int)i);
+        }
+        // segment bookkeeping
+        if (B.segs.empty() || (int)B.segs.size() == cur.s_begin || B.segs.back().dst != T.dst ||
+            B.segs.back().t_end != (int)i) {
+            Segment S;
+            S.dst = T.dst;
+            S.Lt = T.Lt;
+            S.t_begin = (int)i;
+            S.t_end = (int)i + 1;
+            B.segs.push_back(S);
+        } else {
+            B.segs.back().t_end = (int)i + 1;
+        }
+        T.off_d = cur.n_d;
+        T.off_s = cur.n_s;
+        T.off_ax = cur.n_ax;
+        T.off_y = cur.n_y;
+        T.off_z = cur.n_z;
+        cur.n_d += nd;
+        cur.n_s += nsct;
+        cur.n_ax += nax;
+        cur.n_y += ny;
+        cur.n_z += nz;
+        int L = std::max(T.Ls, T.Lt);
+        cur.maxpairs = std::max(cur.maxpairs, (2 * T.K + 1) * (2 * L + 1));
+        cur.maxK = std::max(cur.maxK, T.K);
+        cur.max_nmax = std::max(cur.max_nmax, T.Lt + T.Ls + 2);
+        cur.max_Ls = std::max(cur.max_Ls, T.Ls);
+        cur.max_rows_y = std::max(cur.max_rows_y, rows_size(T.Ls, T.K));
+        cur.max_rows_z = std::max(cur.max_rows_z, rows_size(T.Lt, T.K));
+        int nent = T.Lt * (T.Lt + 2);
+        cur.rowback_blocks = std::max(cur.rowback_blocks, std::min(64, (nent + 255) / 256));
+        used += bytes;
+    }
+    if (n > 0) close_chunk((int)n);
+    // segments were appended in task order; chunk s ranges are consistent by construction
+    // workspace sizes: max over chunks (reused) or sum (cached: single chunk)
+    int64_t md = 0, ms = 0, ma = 0, my = 0, mz = 0;
+    for (auto &ch : B.chunks) {
+        md = std::max(md, ch.n_d);
+        ms = std::max(ms, ch.n_s);
+        ma = std::max(ma, ch.n_ax);
+        my = std::max(my, ch.n_y);
+        mz = std::max(mz, ch.n_z);
+    }
+    int rc;
+    if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, md * 8))) != IFS_OK) return rc;
+    if ((rc = c->ws_s.alloc(std::max<size_t>(c->ws_s.bytes, ms * 16))) != IFS_OK) return rc;
+    if ((rc = c->ws_ax.alloc(std::max<size_t>(c->ws_ax.bytes, ma * 16))) != IFS_OK) return rc;
+    if ((rc = c->ws_y.alloc(std::max<size_t>(c->ws_y.bytes, my * 16))) != IFS_OK) return rc;
+    if ((rc = c->ws_z.alloc(std::max<size_t>(c->ws_z.bytes, mz * 16))) != IFS_OK) return rc;
+    if ((rc = B.d_tasks.alloc(sizeof(Task) * std::max<size_t>(1, n))) != IFS_OK) return rc;
+    if ((rc = B.d_segs.alloc(sizeof(Segment) * std::max<size_t>(1, B.segs.size()))) != IFS_OK) return rc;
+    if (n) HIP_OK(hipMemcpy(B.d_tasks.p, B.tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice));
+    if (!B.segs.empty())
+        HIP_OK(hipMemcpy(B.d_segs.p, B.segs.data(), sizeof(Segment) * B.segs.size(), hipMemcpyHostToDevice));
+    B.tables_ready = false;
+    return IFS_OK;
+}
+
+// Profiling wrapper
+struct Scope {
+    ifs_ctx *c;
+    const char *name;
+    hipEvent_t a{}, b{};
+    Scope(ifs_ctx *c_, const char *n) : c(c_), name(n) {
+        if (c->prof.on) {
+            a = c->prof.get();
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Scope() {
+        if (c->prof.on) {
+            b = c->prof.get();
+            (void)hipEventRecord(b, c->stream);
+            c->prof.recs.push_back({name, a, b});
+        }
+    }
+};
+
+int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) {
+    const Task *dt = B.d_tasks.as<Task>();
+    const Segment *dsg = B.d_segs.as<Segment>();
+    for (auto &ch : B.chunks) {
+        int nt = ch.t_end - ch.t_begin;
+        int nsg = ch.s_end - ch.s_begin;
+        if (nt <= 0) continue;
+        const Task *tt = dt + ch.t_begin;
+        if (!(B.cached && B.tables_ready)) {
+            {
+                Scope s(c, "gen_wigner");
+                launch_gen_wigner(tt, nt, ch.maxpairs, c->ws_d.as<double>(), c->stream);
+            }
+            {
+                Scope s(c, "gen_sect");
+                launch_gen_sect(tt, nt, ch.max_nmax, c->ws_s.as<double2>(), c->stream);
+            }
+            {
+                Scope s(c, "gen_axial");
+                launch_gen_axial(tt, nt, ch.maxK, ch.max_nmax, c->ws_s.as<double2>(), c->ws_ax.as<double2>(),
+                                 c->stream);
+            }
+        }
+        {
+            Scope s(c, "rot_fwd");
+            launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, src_pool, c->ws_d.as<double>(), c->ws_y.as<double2>(),
+                           c->stream);
+        }
+        {
+            Scope s(c, "axial_apply");
+            launch_axial_apply(tt, nt, ch.max_rows_z, c->ws_ax.as<double2>(), c->ws_y.as<double2>(),
+                               c->ws_z.as<double2>(), c->stream);
+        }
+        {
+            Scope s(c, "rot_back");
+            launch_rot_back(dt, dsg + ch.s_begin, nsg, ch.rowback_blocks, c->ws_d.as<double>(), c->ws_z.as<double2>(),
+                            dst_pool, c->stream);
+        }
+        if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
+    }
+    if (B.cached) B.tables_ready = true;
+    return IFS_OK;
+}
+
+void batch_cost(const Batch &B, double &flops, double &bytes) {
+    for (auto &T : B.tasks) {
+        double f = 0.0;
+        for (int n = 1; n <= T.Ls; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+        for (int n = 1; n <= T.Lt; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+        for (int k = -T.K; k <= T.K; ++k) f += 32.0 * (double)ax_block(k, T.Lt, T.Ls);
+        flops += f;
+        bytes += 4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5;
+    }
+}
+
+hipStream_t ctx_stream(ifs_ctx *c) { return c->stream; }
+int ctx_ns(ifs_ctx *c) { return c->ns; }
+int ctx_N(ifs_ctx *c) { return c->N; }
+const double *ctx_pos(ifs_ctx *c) { return c->pos.data(); }
+double ctx_k(ifs_ctx *c) { return c->prm.k; }
+
+int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **w, double2 **r, double2 **part,
+                     double2 **h, int nblocks) {
+    int rc;
+    if ((rc = c->k_V.alloc(16ull * (restart + 1) * n)) || (rc = c->k_w.alloc(16ull * n)) ||
+        (rc = c->k_r.alloc(16ull * n)) || (rc = c->k_part.alloc(16ull * nblocks * (restart + 1))) ||
+        (rc = c->k_h.alloc(16ull * (restart + 1))))
+        return rc;
+    *V = c->k_V.as<double2>();
+    *w = c->k_w.as<double2>();
+    *r = c->k_r.as<double2>();
+    *part = c->k_part.as<double2>();
+    *h = c->k_h.as<double2>();
+    return IFS_OK;
+}
+
+}  // namespace ifs
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
+    if (!out || !cl || !p) return IFS_E_ARG;
+    *out = nullptr;
+    if (cl->n_spheres < 1 || !cl->centres || !cl->radii || !cl->m_rel) return IFS_E_ARG;
+    if (!(p->k > 0.0) || p->n_global < 1) return IFS_E_ARG;
+    if (p->z_selector != IFS_ZSEL_BESSEL_J && p->z_selector != IFS_ZSEL_HANKEL1) return IFS_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IFS_E_NODEVICE;
+
+    ifs_ctx *c = new (std::nothrow) ifs_ctx();
+    if (!c) return IFS_E_NOMEM;
+    auto fail = [&](int rc) {
+        ifs_destroy(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return fail(IFS_E_HIP);
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return fail(IFS_E_HIP);
+    c->stream = c->own_stream;
+    c->prm = *p;
+    c->ns = cl->n_spheres;
+    c->N = p->n_global;
+    const int ns = c->ns, N = c->N;
+    c->pos.assign(cl->centres, cl->centres + 3 * ns);
+    c->radius.assign(cl->radii, cl->radii + ns);
+    c->mrel.resize(ns);
+    c->Lj.resize(ns);
+    c->Leff.resize(ns);
+    c->active.resize(ns);
+    c->off.resize(ns);
+    for (int j = 0; j < ns; ++j) {
+        if (!(c->radius[j] > 0.0)) return fail(IFS_E_ARG);
+        c->mrel[j] = {cl->m_rel[2 * j], cl->m_rel[2 * j + 1]};
+        int L;
+        if (cl->lmax) {
+            L = cl->lmax[j];
+        } else {
+            double x = p->k * c->radius[j];
+            L = (int)std::ceil(x + 4.05 * std::cbrt(x) + 2.0);
+        }
+        if (L < 1) return fail(IFS_E_ARG);
+        c->Lj[j] = L;
+        c->Leff[j] = std::min(L, N);
+        c->active[j] = p->ref_row_zeroing ? (L >= N ? 1 : 0) : 1;
+    }
+    int64_t acc = 0;
+    for (int j = 0; j < ns; ++j) {
+        c->off[j] = acc;
+        acc += 2LL * c->Leff[j] * (c->Leff[j] + 2);
+    }
+    c->ncompact = acc;
+    c->nflat = 2LL * ns * N * (N + 2);
+
+    int rc;
+    if ((rc = c->d_Leff.alloc(4 * ns)) || (rc = c->d_active.alloc(4 * ns)) || (rc = c->d_off.alloc(8 * ns)) ||
+        (rc = c->d_snm.alloc(8ull * N * (N + 2))) || (rc = c->d_an.alloc(16ull * ns * N)) ||
+        (rc = c->d_bn.alloc(16ull * ns * N)) || (rc = c->d_xh.alloc(16ull * c->ncompact)) ||
+        (rc = c->d_acc.alloc(16ull * c->ncompact)))
+        return fail(rc);
+    std::vector<double> snm = make_snm(N);
+    if (hipMemcpy(c->d_Leff.p, c->Leff.data(), 4 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_active.p, c->active.data(), 4 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_off.p, c->off.data(), 8 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_snm.p, snm.data(), 8ull * N * (N + 2), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(IFS_E_HIP);
+
+    // Mie T-matrix on the device (FP64)
+    {
+        DevBuf dx, dm;
+        std::vector<double> xs(ns);
+        std::vector<double2> ms(ns);
+        for (int j = 0; j < ns; ++j) {
+            xs[j] = p->k * c->radius[j];
+            ms[j] = make_double2(c->mrel[j].real(), c->mrel[j].imag());
+        }
+        if ((rc = dx.alloc(8 * ns)) || (rc = dm.alloc(16 * ns))) return fail(rc);
+        if (hipMemcpy(dx.p, xs.data(), 8 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dm.p, ms.data(), 16 * ns, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(IFS_E_HIP);
+        launch_mie(ns, dx.as<double>(), dm.as<double2>(), N, c->d_an.as<double2>(), c->d_bn.as<double2>(),
+                   c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(IFS_E_HIP);
+    }
+
+    // workspace budget: a fraction of free memory
+    size_t freeb = 0, totb = 0;
+    (void)hipMemGetInfo(&freeb, &totb);
+    c->ws_budget = std::min<size_t>(freeb / 4, (size_t)48 << 30);
+
+    if (!p->use_fmm) {
+        // direct all-pairs plan: for each target j, sources l != j in ascending order
+        Batch &B = c->direct;
+        B.name = "direct";
+        B.tasks.reserve((size_t)ns * (ns - 1));
+        for (int j = 0; j < ns; ++j) {
+            if (!c->active[j]) continue;
+            for (int l = 0; l < ns; ++l) {
+                if (l == j) continue;
+                double dv[3] = {c->pos[3 * l] - c->pos[3 * j], c->pos[3 * l + 1] - c->pos[3 * j + 1],
+                                c->pos[3 * l + 2] - c->pos[3 * j + 2]};
+                B.tasks.push_back(
+                    make_task(dv, p->k, p->z_selector, c->Leff[l], c->Leff[j], c->off[l], c->off[j]));
+            }
+        }
+        if ((rc = batch_finalize(c, B, c->ws_budget, true)) != IFS_OK) return fail(rc);
+        batch_cost(B, c->flops_mv, c->bytes_mv);
+    } else {
+        if ((rc = fmm::build(c, &c->fmm)) != IFS_OK) return fail(rc);
+    }
+    for (int j = 0; j < ns; ++j) c->flops_mv += 12.0 * c->Leff[j] * (c->Leff[j] + 2);
+    c->flops_mv += 4.0 * (double)c->nflat;
+    c->bytes_mv += 2.0 * 16.0 * (double)c->nflat;
+    *out = c;
+    return IFS_OK;
+}
+
+int ifs_destroy(ifs_ctx *c) {
+    if (!c) return IFS_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->fmm) fmm::destroy(c->fmm);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return IFS_OK;
+}
+
+int64_t ifs_vector_length(const ifs_ctx *c) { return c ? c->nflat : -1; }
+
+int ifs_set_stream(ifs_ctx *c, void *s) {
+    if (!c) return IFS_E_ARG;
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return IFS_OK;
+}
+
+void *ifs_get_stream(ifs_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
+    if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
+    const double2 *x = (const double2 *)x_dev;
+    double2 *b = (double2 *)b_dev;
+    {
+        Scope s(c, "pack_in");
+        launch_pack_in(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_off.as<int64_t>(), c->d_snm.as<double>(), x,
+                       c->d_xh.as<double2>(), 1, 0, 0, c->stream);
+    }
+    {
+        Scope s(c, "zero");
+        launch_zero(c->d_acc.as<double2>(), c->ncompact, c->stream);
+    }
+    int rc;
+    if (c->fmm) {
+        rc = fmm::run(c, c->fmm, c->d_xh.as<double2>(), c->d_acc.as<double2>());
+    } else {
+        rc = batch_run(c, c->direct, c->d_xh.as<double2>(), c->d_acc.as<double2>());
+    }
+    if (rc != IFS_OK) return rc;
+    {
+        Scope s(c, "finalize");
+        launch_finalize(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_active.as<int32_t>(), c->d_off.as<int64_t>(),
+                        c->d_snm.as<double>(), c->d_an.as<double2>(), c->d_bn.as<double2>(), x,
+                        c->d_acc.as<double2>(), b, 1, 0, 0, c->stream);
+    }
+    if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
+    return IFS_OK;
+}
+
+int ifs_matvec(ifs_ctx *c, const double *x, double *b, int64_t n) {
+    if (!c || !x || !b || n != c->nflat) return IFS_E_ARG;
+    int rc;
+    if ((rc = c->d_x.alloc(16ull * n)) || (rc = c->d_b.alloc(16ull * n))) return rc;
+    HIP_OK(hipMemcpyAsync(c->d_x.p, x, 16ull * n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = ifs_matvec_device(c, c->d_x.p, c->d_b.p, n)) != IFS_OK) return rc;
+    HIP_OK(hipMemcpyAsync(b, c->d_b.p, 16ull * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return IFS_OK;
+}
+
+int ifs_get_mie(ifs_ctx *c, double *an, double *bn) {
+    if (!c || !an || !bn) return IFS_E_ARG;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpy(an, c->d_an.p, 16ull * c->ns * c->N, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(bn, c->d_bn.p, 16ull * c->ns * c->N, hipMemcpyDeviceToHost));
+    return IFS_OK;
+}
+
+int ifs_profile_enable(ifs_ctx *c, int32_t enable) {
+    if (!c) return IFS_E_ARG;
+    if (!enable) c->prof.flush();
+    c->prof.on = enable != 0;
+    if (enable) c->prof.totals.clear();
+    return IFS_OK;
+}
+
+int ifs_profile_read(ifs_ctx *c, const char **names, double *total_ms, int64_t *launches, int32_t cap) {
+    if (!c) return IFS_E_ARG;
+    c->prof.flush();
+    int i = 0;
+    static std::vector<std::string> keep;  // stable storage for returned names
+    for (auto &kv : c->prof.totals) {
+        if (i >= cap) break;
+        bool found = false;
+        for (auto &s : keep)
+            if (s == kv.first) found = true;
+        if (!found) keep.push_back(kv.first);
+        for (auto &s : keep)
+            if (s == kv.first) names[i] = s.c_str();
+        total_ms[i] = kv.second.first;
+        launches[i] = kv.second.second;
+        ++i;
+    }
+    return i;
+}
+
+int ifs_tree_export(ifs_ctx *c, int64_t *uindex_out, int32_t *levels_out) {
+    if (!c || !uindex_out || !levels_out) return IFS_E_ARG;
+    if (!c->fmm) return IFS_E_ARG;
+    return fmm::tree_export(c->fmm, uindex_out, levels_out);
+}
+
+int ifs_fmm_stats(ifs_ctx *c, int64_t counts[6], int32_t levels_L[32]) {
+    if (!c || !counts || !levels_L) return IFS_E_ARG;
+    if (!c->fmm) return IFS_E_ARG;
+    return fmm::stats(c->fmm, counts, levels_L);
+}
+
+int ifs_matvec_cost(ifs_ctx *c, double *flops, double *bytes) {
+    if (!c || !flops || !bytes) return IFS_E_ARG;
+    *flops = c->flops_mv;
+    *bytes = c->bytes_mv;
+    return IFS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,720 @@
+// HIP kernels (gfx950) for the multi-sphere scattering matvec.
+//
+// Translation pipeline per task (source expansion -> target expansion), all FP64:
+//   gen_wigner : Wigner d^n_{km}(theta) tables by the three-term recurrence in n
+//   gen_sect   : sectorial scalar coaxial coefficients (recurrence in m)
+//   gen_axial  : vector coaxial coefficients A_k(n,nu), B_k(n,nu) (recurrence in nu)
+//   rot_fwd    : y = d(theta) diag(e^{i m phi}) x        (forward rotation)
+//   axial_apply: z = [A B; B A] y per coaxial order k
+//   rot_back   : target += diag(e^{-i m phi}) d(theta)^T z, summed over the tasks of
+//                one target in a fixed order (deterministic, no atomics)
+// The algorithm replaces the Gaunt/3j evaluation of Xu's A/B of
+// libmath/src/math/lib_math_vector_spherical_harmonics.f90:1354-1585 by the
+// rotation / coaxial-shift factorisation; both produce the same operator.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "ifs_internal.h"
+
+namespace ifs {
+
+// ------------------------------------------------------------------------
+// complex helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+// i^p * a
+__device__ __forceinline__ double2 ipow_mul(int p, double2 a) {
+    switch (p & 3) {
+    case 0: return a;
+    case 1: return make_double2(-a.y, a.x);
+    case 2: return make_double2(-a.x, -a.y);
+    default: return make_double2(a.y, -a.x);
+    }
+}
+__device__ __forceinline__ int fidx(int n, int m) { return n * n - 1 + m + n; }
+
+// a^m_n = sqrt((n+1+|m|)(n+1-|m|)/((2n+1)(2n+3))), 0 for n < |m|
+__device__ __forceinline__ double coef_a(int n, int m) {
+    int am = m < 0 ? -m : m;
+    if (n < am) return 0.0;
+    return sqrt((double)(n + 1 + am) * (double)(n + 1 - am) / ((double)(2 * n + 1) * (double)(2 * n + 3)));
+}
+// b^m_n = sign(m) sqrt((n-m-1)(n-m)/((2n-1)(2n+1))) for |m| <= n, 0 otherwise (sign(0) = +)
+__device__ __forceinline__ double coef_b(int n, int m) {
+    int am = m < 0 ? -m : m;
+    if (am > n) return 0.0;
+    double p = (double)(n - m - 1) * (double)(n - m);
+    if (p <= 0.0) return 0.0;
+    double v = sqrt(p / ((double)(2 * n - 1) * (double)(2 * n + 1)));
+    return m >= 0 ? v : -v;
+}
+
+// ------------------------------------------------------------------------
+// Wigner d tables
+// ------------------------------------------------------------------------
+// d^{n0}_{km}(beta) at n0 = max(|k|,|m|) from the closed forms
+//   d^j_{j,m}  = sqrt((2j)!/((j+m)!(j-m)!)) c^{j+m} (-s)^{j-m}
+//   d^j_{-j,m} = sqrt((2j)!/((j+m)!(j-m)!)) c^{j-m} s^{j+m}
+//   d^j_{k,m}  = (-1)^{k-m} d^j_{m,k}
+// with c = cos(beta/2), s = sin(beta/2), evaluated in logs.
+__device__ double wigner_seed(int k, int m, double c, double s) {
+    double sign = 1.0;
+    if ((k < 0 ? -k : k) < (m < 0 ? -m : m)) {
+        int t = k;
+        k = m;
+        m = t;
+        if ((k - m) & 1) sign = -sign;
+    }
+    int j = k < 0 ? -k : k;
+    int ec, es;
+    if (k >= 0) {
+        ec = j + m;
+        es = j - m;
+        if (es & 1) sign = -sign;
+    } else {
+        ec = j - m;
+        es = j + m;
+    }
+    double lv = 0.5 * (lgamma(2.0 * j + 1.0) - lgamma((double)(j + m) + 1.0) - lgamma((double)(j - m) + 1.0));
+    if (ec > 0) {
+        if (c <= 0.0) return 0.0;
+        lv += ec * log(c);
+    }
+    if (es > 0) {
+        if (s <= 0.0) return 0.0;
+        lv += es * log(s);
+    }
+    return sign * exp(lv);
+}
+
+// One workgroup strip per task (grid.y), threads over (k,m) pairs.
+__global__ void k_gen_wigner(const Task *__restrict__ tasks, double *__restrict__ wd) {
+    const Task T = tasks[blockIdx.y];
+    const int K = T.K;
+    const int L = T.Ls > T.Lt ? T.Ls : T.Lt;
+    const int width = 2 * L + 1;
+    const int npairs = (2 * K + 1) * width;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const int k = p / width - K;
+    const int m = p % width - L;
+    const int ak = k < 0 ? -k : k, am = m < 0 ? -m : m;
+    const int n0 = ak > am ? ak : am;
+    const double cb = T.cth;
+    const double sb = sqrt(fmax(0.0, 1.0 - cb * cb));
+    // half-angle functions from cos(beta), beta in [0, pi]
+    const double ch = sqrt(fmax(0.0, 0.5 * (1.0 + cb)));
+    const double sh = sqrt(fmax(0.0, 0.5 * (1.0 - cb)));
+    (void)sb;
+    double *base = wd + T.off_d;
+    double dm2, dm1;
+    int nstart;
+    if (n0 == 0) {
+        dm2 = 1.0;
+        dm1 = cb;  // d^1_{00}
+        if (L >= 1) base[wig_off(1, K) + (0 + (1 < K ? 1 : K)) * 3 + 1] = cb;
+        nstart = 2;
+    } else {
+        dm2 = 0.0;
+        dm1 = wigner_seed(k, m, ch, sh);
+        int kk = n0 < K ? n0 : K;
+        base[wig_off(n0, K) + (int64_t)(k + kk) * (2 * n0 + 1) + (m + n0)] = dm1;
+        nstart = n0 + 1;
+    }
+    const double km = (double)k * (double)m;
+    const double k2 = (double)k * k, m2 = (double)m * m;
+    for (int n = nstart; n <= L; ++n) {
+        double nn = (double)n;
+        double num = (2.0 * nn - 1.0) * (nn * (nn - 1.0) * cb - km) * dm1 -
+                     nn * sqrt(((nn - 1.0) * (nn - 1.0) - k2) * ((nn - 1.0) * (nn - 1.0) - m2)) * dm2;
+        double den = (nn - 1.0) * sqrt((nn * nn - k2) * (nn * nn - m2));
+        double dn = num / den;
+        int kk = n < K ? n : K;
+        base[wig_off(n, K) + (int64_t)(k + kk) * (2 * n + 1) + (m + n)] = dn;
+        dm2 = dm1;
+        dm1 = dn;
+    }
+}
+
+void launch_gen_wigner(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s) {
+    if (ntasks <= 0) return;
+    dim3 grid((maxpairs + 255) / 256, ntasks);
+    hipLaunchKernelGGL(k_gen_wigner, grid, dim3(256), 0, s, tasks, wd);
+}
+
+// ------------------------------------------------------------------------
+// Radial functions (one thread)
+// ------------------------------------------------------------------------
+// j_n(x), n = 0..nmax, by Miller's downward recurrence written straight into
+// out[n].x, normalised with j_0 or j_1; y_n upward into out[n].y (kind 3).
+__device__ void radial_into(double x, int nmax, int kind, double2 *out) {
+    if (x == 0.0) {
+        for (int n = 0; n <= nmax; ++n) out[n] = make_double2(n == 0 ? 1.0 : 0.0, 0.0);
+        return;
+    }
+    const double big = (double)nmax > x ? (double)nmax : x;
+    const int nstart = (int)big + 30 + (int)sqrt(40.0 * big);
+    double fp1 = 0.0, f = 1e-200;  // f_{n+1}, f_n
+    for (int n = nstart; n >= 1; --n) {
+        double fm1 = (2.0 * n + 1.0) / x * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (n - 1 <= nmax) out[n - 1].x = f;
+        if (fabs(f) > 1e200) {
+            f *= 1e-200;
+            fp1 *= 1e-200;
+            if (n - 1 <= nmax)
+                for (int q = n - 1; q <= nmax; ++q) out[q].x *= 1e-200;
+        }
+    }
+    const double j0 = sin(x) / x;
+    const double j1 = sin(x) / (x * x) - cos(x) / x;
+    const double scale = (nmax >= 1 && fabs(j1) > fabs(j0)) ? j1 / out[1].x : j0 / out[0].x;
+    for (int n = 0; n <= nmax; ++n) out[n].x *= scale;
+    if (kind == 3) {
+        double ym1 = -cos(x) / x;
+        out[0].y = ym1;
+        if (nmax >= 1) {
+            double y = -cos(x) / (x * x) - sin(x) / x;
+            out[1].y = y;
+            for (int n = 1; n < nmax; ++n) {
+                double yp = (2.0 * n + 1.0) / x * y - ym1;
+                ym1 = y;
+                y = yp;
+                out[n + 1].y = y;
+            }
+        }
+    } else {
+        for (int n = 0; n <= nmax; ++n) out[n].y = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Sectorial scalar coaxial columns alpha^m(n, nu = m), m = 0..K
+//   alpha^0(n,0)     = (-1)^n sqrt(2n+1) f_n(t)
+//   alpha^{m+1}(n,m+1) = [b^{-m-1}_n alpha^m(n-1,m) - b^m_{n+1} alpha^m(n+1,m)] / b^{-m-1}_{m+1}
+// Column m is valid for n in [m, nmax - m]; stored with stride nmax+2.
+// ------------------------------------------------------------------------
+__global__ void k_gen_sect(const Task *__restrict__ tasks, int max_nmax, double2 *__restrict__ wsect) {
+    extern __shared__ double2 sh[];
+    const Task T = tasks[blockIdx.x];
+    const int nmax = T.Lt + T.Ls + 2;
+    double2 *cur = sh;
+    double2 *nxt = sh + (max_nmax + 2);
+    if (threadIdx.x == 0) radial_into(T.t, nmax, T.kind, cur);
+    __syncthreads();
+    for (int n = threadIdx.x; n <= nmax; n += blockDim.x) {
+        double s = sqrt(2.0 * n + 1.0) * ((n & 1) ? -1.0 : 1.0);
+        cur[n] = cscale(s, cur[n]);
+    }
+    __syncthreads();
+    double2 *out = wsect + T.off_s;
+    const int stride = nmax + 2;
+    for (int m = 0; m <= T.K; ++m) {
+        const int top = nmax - m;
+        for (int n = m + threadIdx.x; n <= top; n += blockDim.x) out[(int64_t)m * stride + n] = cur[n];
+        if (m == T.K) break;
+        const double den = coef_b(m + 1, -m - 1);
+        for (int n = m + 1 + threadIdx.x; n <= top - 1; n += blockDim.x) {
+            double2 lo = cscale(coef_b(n, -m - 1), cur[n - 1]);
+            double2 hi = cscale(coef_b(n + 1, m), cur[n + 1]);
+            nxt[n] = cscale(1.0 / den, csub(lo, hi));
+        }
+        __syncthreads();
+        double2 *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+}
+
+void launch_gen_sect(const Task *tasks, int ntasks, int max_nmax, double2 *wsect, hipStream_t s) {
+    if (ntasks <= 0) return;
+    size_t shm = sizeof(double2) * 2 * (size_t)(max_nmax + 2);
+    hipLaunchKernelGGL(k_gen_sect, dim3(ntasks), dim3(256), shm, s, tasks, max_nmax, wsect);
+}
+
+// ------------------------------------------------------------------------
+// Vector coaxial tables for one (task, m >= 0): recurrence in nu
+//   alpha(n,nu+1) = [a_{nu-1} alpha(n,nu-1) - a_n alpha(n+1,nu) + a_{n-1} alpha(n-1,nu)] / a_nu
+// and the vector relation (normalised basis)
+//   A_m(n,nu) = i^{nu-n} [sqrt(n(n+1)) al(n,nu) + t a_n sqrt(n/(n+1)) al(n+1,nu)
+//                         + t a_{n-1} sqrt((n+1)/n) al(n-1,nu)] / sqrt(nu(nu+1))
+//   B_m(n,nu) = i^{nu-n+1} t m al(n,nu) / sqrt(n(n+1) nu(nu+1))
+// Order -m shares A and has B negated.
+// ------------------------------------------------------------------------
+__global__ void k_gen_axial(const Task *__restrict__ tasks, int max_nmax, const double2 *__restrict__ wsect,
+                            double2 *__restrict__ wax) {
+    extern __shared__ double2 sh[];
+    const Task T = tasks[blockIdx.y];
+    const int m = blockIdx.x;
+    if (m > T.K) return;
+    const int nmax = T.Lt + T.Ls + 2;
+    const int Lt = T.Lt, Ls = T.Ls, K = T.K;
+    const double t = T.t;
+    double2 *P = sh;
+    double2 *C = sh + (max_nmax + 2);
+    double2 *N = sh + 2 * (max_nmax + 2);
+    const double2 *src = wsect + T.off_s + (int64_t)m * (nmax + 2);
+    int top = nmax - m;
+    for (int n = threadIdx.x; n <= nmax + 1; n += blockDim.x) {
+        P[n] = make_double2(0.0, 0.0);
+        C[n] = (n >= m && n <= top) ? src[n] : make_double2(0.0, 0.0);
+        N[n] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int q = m < 1 ? 1 : m;
+    double2 *Ap = wax + T.off_ax + ax_off(m, Lt, Ls, K);
+    double2 *Bp = Ap + ax_block(m, Lt, Ls);
+    double2 *An = wax + T.off_ax + ax_off(-m, Lt, Ls, K);
+    double2 *Bn = An + ax_block(-m, Lt, Ls);
+    const int ncol = Ls - q + 1;
+    for (int nu = m; nu <= Ls; ++nu) {
+        if (nu >= 1) {
+            const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1));
+            for (int n = q + threadIdx.x; n <= Lt; n += blockDim.x) {
+                double nn = (double)n;
+                double2 c0 = C[n];
+                double2 cp = C[n + 1];
+                double2 cm = (n - 1 >= m) ? C[n - 1] : make_double2(0.0, 0.0);
+                double2 s = cscale(sqrt(nn * (nn + 1.0)), c0);
+                s = cadd(s, cscale(t * coef_a(n, m) * sqrt(nn / (nn + 1.0)), cp));
+                s = cadd(s, cscale(t * coef_a(n - 1, m) * sqrt((nn + 1.0) / nn), cm));
+                double2 av = ipow_mul((nu - n) & 3, cscale(inv_nu, s));
+                double2 bv = ipow_mul((nu - n + 1) & 3, cscale(t * (double)m * inv_nu / sqrt(nn * (nn + 1.0)), c0));
+                int64_t o = (int64_t)(n - q) * ncol + (nu - q);
+                Ap[o] = av;
+                Bp[o] = bv;
+                if (m > 0) {
+                    An[o] = av;
+                    Bn[o] = make_double2(-bv.x, -bv.y);
+                }
+            }
+        }
+        if (nu == Ls) break;
+        // advance nu -> nu+1 on rows [m, top-1]
+        const double anu = coef_a(nu, m);
+        const double anum1 = coef_a(nu - 1, m);
+        for (int n = m + threadIdx.x; n <= top - 1; n += blockDim.x) {
+            double2 v = cscale(anum1, P[n]);
+            v = csub(v, cscale(coef_a(n, m), C[n + 1]));
+            if (n - 1 >= m) v = cadd(v, cscale(coef_a(n - 1, m), C[n - 1]));
+            N[n] = cscale(1.0 / anu, v);
+        }
+        __syncthreads();
+        // rotate P <- C <- N
+        double2 *tmp = P;
+        P = C;
+        C = N;
+        N = tmp;
+        top -= 1;
+        for (int n = top + 1 + threadIdx.x; n <= nmax + 1; n += blockDim.x) C[n] = make_double2(0.0, 0.0);
+        __syncthreads();
+    }
+}
+
+void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect, double2 *wax,
+                      hipStream_t s) {
+    if (ntasks <= 0) return;
+    size_t shm = sizeof(double2) * 3 * (size_t)(max_nmax + 2);
+    hipLaunchKernelGGL(k_gen_axial, dim3(maxK + 1, ntasks), dim3(256), shm, s, tasks, max_nmax, wsect, wax);
+}
+
+// ------------------------------------------------------------------------
+// Forward rotation: y(nu,k) = sum_m d^nu_{km} e^{i m phi} x(nu,m), |k| <= min(nu,K)
+// ------------------------------------------------------------------------
+__global__ void k_rot_fwd(const Task *__restrict__ tasks, const double2 *__restrict__ src_pool,
+                          const double *__restrict__ wd, double2 *__restrict__ wy) {
+    extern __shared__ double2 ph[];
+    const Task T = tasks[blockIdx.y];
+    const int Ls = T.Ls, K = T.K;
+    for (int m = threadIdx.x; m <= 2 * Ls; m += blockDim.x) {
+        double sn, cs;
+        sincos((double)(m - Ls) * T.phi, &sn, &cs);
+        ph[m] = make_double2(cs, sn);
+    }
+    __syncthreads();
+    const int64_t nent = rows_size(Ls, K);
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    int nu, k;
+    {
+        int64_t lim = (int64_t)(K + 1) * (K + 1) - 1;
+        if (e < lim) {
+            nu = (int)sqrt((double)(e + 1));
+            while ((int64_t)(nu + 1) * (nu + 1) - 1 <= e) ++nu;
+            while ((int64_t)nu * nu - 1 > e) --nu;
+        } else {
+            nu = K + 1 + (int)((e - lim) / (2 * K + 1));
+        }
+        int kk = nu < K ? nu : K;
+        k = (int)(e - rows_off(nu, K)) - kk;
+    }
+    const int kk = nu < K ? nu : K;
+    const double *drow = wd + T.off_d + wig_off(nu, K) + (int64_t)(k + kk) * (2 * nu + 1);
+    const double2 *xa = src_pool + T.src + fidx(nu, 0) - nu;
+    const double2 *xb = xa + (int64_t)Ls * (Ls + 2);
+    double2 sa = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
+    for (int m = -nu; m <= nu; ++m) {
+        double d = drow[m + nu];
+        double2 e_ = ph[m + Ls];
+        double2 va = cmul(e_, xa[m + nu]);
+        double2 vb = cmul(e_, xb[m + nu]);
+        sa.x += d * va.x;
+        sa.y += d * va.y;
+        sb.x += d * vb.x;
+        sb.y += d * vb.y;
+    }
+    double2 *y = wy + T.off_y;
+    y[e] = sa;
+    y[e + nent] = sb;
+}
+
+void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls, const double2 *src_pool,
+                    const double *wd, double2 *wy, hipStream_t s) {
+    if (ntasks <= 0) return;
+    dim3 grid((unsigned)((max_rows + 255) / 256), ntasks);
+    size_t shm = sizeof(double2) * (size_t)(2 * max_Ls + 1);
+    hipLaunchKernelGGL(k_rot_fwd, grid, dim3(256), shm, s, tasks, src_pool, wd, wy);
+}
+
+// ------------------------------------------------------------------------
+// Coaxial shift: z(n,k) = sum_nu A_k(n,nu) y_a(nu,k) + B_k(n,nu) y_b(nu,k)  (a-part)
+//                z_b    = sum_nu B_k(n,nu) y_a + A_k(n,nu) y_b
+// ------------------------------------------------------------------------
+__global__ void k_axial_apply(const Task *__restrict__ tasks, const double2 *__restrict__ wax,
+                              const double2 *__restrict__ wy, double2 *__restrict__ wz) {
+    const Task T = tasks[blockIdx.y];
+    const int Lt = T.Lt, Ls = T.Ls, K = T.K;
+    const int64_t nent = rows_size(Lt, K);
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    int n, k;
+    {
+        int64_t lim = (int64_t)(K + 1) * (K + 1) - 1;
+        if (e < lim) {
+            n = (int)sqrt((double)(e + 1));
+            while ((int64_t)(n + 1) * (n + 1) - 1 <= e) ++n;
+            while ((int64_t)n * n - 1 > e) --n;
+        } else {
+            n = K + 1 + (int)((e - lim) / (2 * K + 1));
+        }
+        int kk = n < K ? n : K;
+        k = (int)(e - rows_off(n, K)) - kk;
+    }
+    const int q = (k < 0 ? -k : k) < 1 ? 1 : (k < 0 ? -k : k);
+    const int ncol = Ls - q + 1;
+    const double2 *Ab = wax + T.off_ax + ax_off(k, Lt, Ls, K) + (int64_t)(n - q) * ncol;
+    const double2 *Bb = Ab + ax_block(k, Lt, Ls);
+    const int64_t ny = rows_size(Ls, K);
+    const double2 *ya = wy + T.off_y;
+    const double2 *yb = ya + ny;
+    double2 za = make_double2(0.0, 0.0), zb = make_double2(0.0, 0.0);
+    for (int nu = q; nu <= Ls; ++nu) {
+        int kk = nu < K ? nu : K;
+        int64_t iy = rows_off(nu, K) + k + kk;
+        double2 a = Ab[nu - q], b = Bb[nu - q];
+        double2 va = ya[iy], vb = yb[iy];
+        za = cadd(za, cadd(cmul(a, va), cmul(b, vb)));
+        zb = cadd(zb, cadd(cmul(b, va), cmul(a, vb)));
+    }
+    double2 *z = wz + T.off_z;
+    z[e] = za;
+    z[e + nent] = zb;
+}
+
+void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const double2 *wax, const double2 *wy,
+                        double2 *wz, hipStream_t s) {
+    if (ntasks <= 0) return;
+    dim3 grid((unsigned)((max_rows + 255) / 256), ntasks);
+    hipLaunchKernelGGL(k_axial_apply, grid, dim3(256), 0, s, tasks, wax, wy, wz);
+}
+
+// ------------------------------------------------------------------------
+// Back rotation and accumulation into the target, one workgroup per target segment:
+//   out(n,m) += e^{-i m phi} sum_k d^n_{km} z(n,k)
+// ------------------------------------------------------------------------
+__global__ void k_rot_back(const Task *__restrict__ tasks, const Segment *__restrict__ segs,
+                           const double *__restrict__ wd, const double2 *__restrict__ wz,
+                           double2 *__restrict__ dst_pool) {
+    const Segment S = segs[blockIdx.x];
+    const int Lt = S.Lt;
+    const int nent = Lt * (Lt + 2);
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < nent; e += blockDim.x * gridDim.y) {
+        int n = (int)sqrt((double)(e + 1));
+        while ((n + 1) * (n + 1) - 1 <= e) ++n;
+        while (n * n - 1 > e) --n;
+        const int m = e - (n * n - 1) - n;
+        double2 oa = make_double2(0.0, 0.0), ob = make_double2(0.0, 0.0);
+        for (int ti = S.t_begin; ti < S.t_end; ++ti) {
+            const Task T = tasks[ti];
+            const int K = T.K;
+            const int kk = n < K ? n : K;
+            const double *dcol = wd + T.off_d + wig_off(n, K) + (m + n);
+            const int64_t nz = rows_size(Lt, K);
+            const double2 *za = wz + T.off_z + rows_off(n, K) + kk;
+            const double2 *zb = za + nz;
+            double2 sa = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
+            for (int k = -kk; k <= kk; ++k) {
+                double d = dcol[(int64_t)(k + kk) * (2 * n + 1)];
+                sa.x += d * za[k].x;
+                sa.y += d * za[k].y;
+                sb.x += d * zb[k].x;
+                sb.y += d * zb[k].y;
+            }
+            double sn, cs;
+            sincos(-(double)m * T.phi, &sn, &cs);
+            double2 e_ = make_double2(cs, sn);
+            oa = cadd(oa, cmul(e_, sa));
+            ob = cadd(ob, cmul(e_, sb));
+        }
+        double2 *dst = dst_pool + S.dst;
+        dst[e] = cadd(dst[e], oa);
+        dst[e + nent] = cadd(dst[e + nent], ob);
+    }
+}
+
+void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblocks, const double *wd,
+                     const double2 *wz, double2 *dst_pool, hipStream_t s) {
+    if (nsegs <= 0) return;
+    dim3 grid(nsegs, yblocks < 1 ? 1 : yblocks);
+    hipLaunchKernelGGL(k_rot_back, grid, dim3(256), 0, s, tasks, segs, wd, wz, dst_pool);
+}
+
+}  // namespace ifs
+
+namespace ifs {
+
+// ------------------------------------------------------------------------
+// Mie coefficients a_n, b_n (Barber & Hill log-derivative form), one block per
+// sphere, sequential recurrences on thread 0 (FP64).  Restates
+// FoggySim/src/lib/mie_theory/lib_mie_ss_helper_functions.f90:610-722 (a_n, b_n),
+// :819-865 and :900-924 (A_n downward), :937-951 (n_c).
+// ------------------------------------------------------------------------
+__global__ void k_mie(const double *__restrict__ xsize, const double2 *__restrict__ mrel, int N,
+                      double2 *__restrict__ an, double2 *__restrict__ bn) {
+    extern __shared__ double2 shm[];
+    if (threadIdx.x != 0) return;
+    const int j = blockIdx.x;
+    const double x = xsize[j];
+    const double2 m = mrel[j];
+    double2 *An = shm;            // [N+2]
+    double2 *f = shm + (N + 2);   // [N+1]: j_n in .x, y_n in .y
+    const double2 mx = make_double2(m.x * x, m.y * x);
+    const double amx = sqrt(mx.x * mx.x + mx.y * mx.y);
+    const int nc = (int)ceil(x + 4.05 * cbrt(x) + 2.0);
+    int nstart = (nc > (int)amx ? nc : (int)amx) + 15;
+    if (nstart < N + 1) nstart = N + 1;  // every A_n, n <= N, comes from the recurrence
+    double2 buf = make_double2(0.0, 0.0);
+    for (int i = nstart; i >= 2; --i) {
+        double2 ndm = cdiv(make_double2((double)i, 0.0), mx);
+        buf = csub(ndm, cdiv(make_double2(1.0, 0.0), cadd(buf, ndm)));
+        if (i - 1 <= N) An[i - 1] = buf;
+    }
+    radial_into(x, N, 3, f);
+    for (int n = 1; n <= N; ++n) {
+        double2 jn = make_double2(f[n].x, 0.0), jn1 = make_double2(f[n - 1].x, 0.0);
+        double2 hn = f[n], hn1 = f[n - 1];
+        double2 ndx = make_double2((double)n / x, 0.0);
+        double2 t = cadd(cdiv(An[n], m), ndx);
+        an[(int64_t)j * N + n - 1] = cdiv(csub(cmul(t, jn), jn1), csub(cmul(t, hn), hn1));
+        t = cadd(cmul(m, An[n]), ndx);
+        bn[(int64_t)j * N + n - 1] = cdiv(csub(cmul(t, jn), jn1), csub(cmul(t, hn), hn1));
+    }
+}
+
+void launch_mie(int ns, const double *xsize, const double2 *mrel, int N, double2 *an, double2 *bn, hipStream_t s) {
+    if (ns <= 0) return;
+    size_t shm = sizeof(double2) * (size_t)(2 * N + 4);
+    hipLaunchKernelGGL(k_mie, dim3(ns), dim3(64), shm, s, xsize, mrel, N, an, bn);
+}
+
+// ------------------------------------------------------------------------
+// Layout conversion.  Reference flat layout (Xu basis, global N) <-> compact
+// normalised per-sphere blocks (order Leff_j).
+// ------------------------------------------------------------------------
+__global__ void k_pack_in(int N, const int32_t *__restrict__ Leff, const int64_t *__restrict__ off,
+                          const double *__restrict__ snm, const double2 *__restrict__ x, double2 *__restrict__ xh,
+                          int64_t ldx, int64_t ldxh) {
+    const int j = blockIdx.x;
+    const int r = blockIdx.z;
+    const int L = Leff[j];
+    const int blk = N * (N + 2);
+    const int cnt = L * (L + 2);
+    const double2 *xs = x + r * ldx + (int64_t)j * 2 * blk;
+    double2 *xd = xh + r * ldxh + off[j];
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < cnt; e += blockDim.x * gridDim.y) {
+        double s = snm[e];
+        xd[e] = cscale(s, xs[e]);
+        xd[e + cnt] = cscale(s, xs[e + blk]);
+    }
+}
+
+void launch_pack_in(int ns, int N, const int32_t *Leff, const int64_t *off, const double *snm, const double2 *x,
+                    double2 *xh, int nrhs, int64_t ldx, int64_t ldxh, hipStream_t s) {
+    if (ns <= 0) return;
+    int yb = (N * (N + 2) + 255) / 256;
+    hipLaunchKernelGGL(k_pack_in, dim3(ns, yb, nrhs), dim3(256), 0, s, N, Leff, off, snm, x, xh, ldx, ldxh);
+}
+
+// b = x + [active_j, n <= Leff_j] T_n(j) acc(n,m) / s(n,m)
+__global__ void k_finalize(int N, const int32_t *__restrict__ Leff, const int32_t *__restrict__ active,
+                           const int64_t *__restrict__ off, const double *__restrict__ snm,
+                           const double2 *__restrict__ an, const double2 *__restrict__ bn,
+                           const double2 *__restrict__ x, const double2 *__restrict__ acc, double2 *__restrict__ b,
+                           int64_t ldx, int64_t ldacc) {
+    const int j = blockIdx.x;
+    const int r = blockIdx.z;
+    const int L = Leff[j];
+    const int blk = N * (N + 2);
+    const int cnt = L * (L + 2);
+    const double2 *xs = x + r * ldx + (int64_t)j * 2 * blk;
+    double2 *bs = b + r * ldx + (int64_t)j * 2 * blk;
+    const double2 *as = acc + r * ldacc + off[j];
+    const int act = active[j];
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < blk; e += blockDim.x * gridDim.y) {
+        double2 va = xs[e], vb = xs[e + blk];
+        if (act && e < cnt) {
+            int n = (int)sqrt((double)(e + 1));
+            while ((n + 1) * (n + 1) - 1 <= e) ++n;
+            while (n * n - 1 > e) --n;
+            double inv = 1.0 / snm[e];
+            va = cadd(va, cmul(an[(int64_t)j * N + n - 1], cscale(inv, as[e])));
+            vb = cadd(vb, cmul(bn[(int64_t)j * N + n - 1], cscale(inv, as[e + cnt])));
+        }
+        bs[e] = va;
+        bs[e + blk] = vb;
+    }
+}
+
+void launch_finalize(int ns, int N, const int32_t *Leff, const int32_t *active, const int64_t *off,
+                     const double *snm, const double2 *an, const double2 *bn, const double2 *x, const double2 *acc,
+                     double2 *b, int nrhs, int64_t ldx, int64_t ldacc, hipStream_t s) {
+    if (ns <= 0) return;
+    int yb = (N * (N + 2) + 255) / 256;
+    hipLaunchKernelGGL(k_finalize, dim3(ns, yb, nrhs), dim3(256), 0, s, N, Leff, active, off, snm, an, bn, x, acc,
+                       b, ldx, ldacc);
+}
+
+__global__ void k_zero(double2 *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)blockDim.x * gridDim.x)
+        p[i] = make_double2(0.0, 0.0);
+}
+
+void launch_zero(double2 *p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+}
+
+// ------------------------------------------------------------------------
+// Krylov BLAS-1 kernels (FP64 complex), fixed reduction order.
+// ------------------------------------------------------------------------
+// partial[b * nv + i] = sum over block b's slice of conj(V_i) w ; then out[i] = sum_b partial
+__global__ void k_dots_partial(const double2 *__restrict__ V, int64_t ldv, int nv, const double2 *__restrict__ w,
+                               int64_t n, double2 *__restrict__ partial) {
+    __shared__ double2 red[256];
+    for (int i = 0; i < nv; ++i) {
+        const double2 *v = V + (int64_t)i * ldv;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)blockDim.x * gridDim.x) {
+            double2 a = v[e], b = w[e];
+            acc.x += a.x * b.x + a.y * b.y;
+            acc.y += a.x * b.y - a.y * b.x;
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + i] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_dots_final(const double2 *__restrict__ partial, int nblocks, int nv, double2 *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = 0; b < nblocks; ++b) s = cadd(s, partial[(int64_t)b * nv + i]);
+    out[i] = s;
+}
+
+void launch_dots(const double2 *V, int64_t ldv, int nv, const double2 *w, int64_t n, double2 *out, double2 *partial,
+                 int nblocks, hipStream_t s) {
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_dots_partial, dim3(nblocks), dim3(256), 0, s, V, ldv, nv, w, n, partial);
+    hipLaunchKernelGGL(k_dots_final, dim3((nv + 63) / 64), dim3(64), 0, s, partial, nblocks, nv, out);
+}
+
+// w -= sum_i h_i V_i
+__global__ void k_gemv_sub(const double2 *__restrict__ V, int64_t ldv, int nv, const double2 *__restrict__ h,
+                           double2 *__restrict__ w, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)blockDim.x * gridDim.x) {
+        double2 acc = w[e];
+        for (int i = 0; i < nv; ++i) acc = csub(acc, cmul(h[i], V[(int64_t)i * ldv + e]));
+        w[e] = acc;
+    }
+}
+
+void launch_gemv_sub(const double2 *V, int64_t ldv, int nv, const double2 *h, double2 *w, int64_t n, hipStream_t s) {
+    if (nv <= 0) return;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_gemv_sub, dim3((unsigned)nb), dim3(256), 0, s, V, ldv, nv, h, w, n);
+}
+
+// x += sum_i c_i V_i
+__global__ void k_gemv_add(const double2 *__restrict__ V, int64_t ldv, int nv, const double2 *__restrict__ c,
+                           double2 *__restrict__ x, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)blockDim.x * gridDim.x) {
+        double2 acc = x[e];
+        for (int i = 0; i < nv; ++i) acc = cadd(acc, cmul(c[i], V[(int64_t)i * ldv + e]));
+        x[e] = acc;
+    }
+}
+
+void launch_gemv_add(const double2 *V, int64_t ldv, int nv, const double2 *c, double2 *x, int64_t n, hipStream_t s) {
+    if (nv <= 0) return;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_gemv_add, dim3((unsigned)nb), dim3(256), 0, s, V, ldv, nv, c, x, n);
+}
+
+// y = a x + b y
+__global__ void k_axpby(double2 a, const double2 *__restrict__ x, double2 b, double2 *__restrict__ y, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)blockDim.x * gridDim.x)
+        y[e] = cadd(cmul(a, x[e]), cmul(b, y[e]));
+}
+
+void launch_axpby(double2 a, const double2 *x, double2 b, double2 *y, int64_t n, hipStream_t s) {
+    int64_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) return;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)nb), dim3(256), 0, s, a, x, b, y, n);
+}
+
+// y = a x
+__global__ void k_scale_copy(double2 a, const double2 *__restrict__ x, double2 *__restrict__ y, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)blockDim.x * gridDim.x)
+        y[e] = cmul(a, x[e]);
+}
+
+void launch_scale_copy(double2 a, const double2 *x, double2 *y, int64_t n, hipStream_t s) {
+    int64_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) return;
+    hipLaunchKernelGGL(k_scale_copy, dim3((unsigned)nb), dim3(256), 0, s, a, x, y, n);
+}
+
+}  // namespace ifs

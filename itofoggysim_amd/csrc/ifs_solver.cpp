@@ -1,0 +1,236 @@
+// Restarted GMRES on the device (FP64), the plane-wave right-hand side, and the
+// remaining C ABI entry points.
+//
+// The reference drives CERFACS ZGMRES by reverse communication with modified
+// Gram-Schmidt and zgemv dot products (FoggySim/src/lib/mie_theory/lib_solver_GMRES.f90:116-316).
+// Here the Krylov basis lives in HBM; orthogonalisation is classical Gram-Schmidt
+// with one re-orthogonalisation pass (batched dot products, fixed reduction
+// order), and the small Hessenberg / Givens update runs on the host in FP64.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "../../include/ifs.h"
+#include "ifs_internal.h"
+
+using namespace ifs;
+using cplx = std::complex<double>;
+
+// Accessors implemented in ifs_api.cpp
+namespace ifs {
+hipStream_t ctx_stream(ifs_ctx *c);
+int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **w, double2 **r, double2 **part,
+                     double2 **h, int nblocks);
+int ctx_ns(ifs_ctx *c);
+int ctx_N(ifs_ctx *c);
+const double *ctx_pos(ifs_ctx *c);
+double ctx_k(ifs_ctx *c);
+}  // namespace ifs
+
+namespace {
+
+constexpr int kDotBlocks = 512;
+
+int dots(hipStream_t s, const double2 *V, int64_t ldv, int nv, const double2 *w, int64_t n, double2 *d_out,
+         double2 *d_part, std::vector<cplx> &out) {
+    launch_dots(V, ldv, nv, w, n, d_out, d_part, kDotBlocks, s);
+    out.resize(nv);
+    if (hipMemcpyAsync(out.data(), d_out, sizeof(double2) * nv, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return IFS_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return IFS_E_HIP;
+    return IFS_OK;
+}
+
+double norm2(hipStream_t s, const double2 *w, int64_t n, double2 *d_out, double2 *d_part, int &rc) {
+    std::vector<cplx> o;
+    rc = dots(s, w, n, 1, w, n, d_out, d_part, o);
+    return std::sqrt(std::max(0.0, o[0].real()));
+}
+
+}  // namespace
+
+extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, int64_t n, const ifs_solve_opts *opts,
+                                ifs_solve_info *info) {
+    if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c)) return IFS_E_ARG;
+    const int m = opts->restart;
+    if (m < 1 || opts->max_iter < 1 || !(opts->tol > 0.0)) return IFS_E_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = ctx_stream(c);
+    double2 *V, *w, *r, *part, *h;
+    int rc = ctx_krylov_alloc(c, m, n, &V, &w, &r, &part, &h, kDotBlocks);
+    if (rc) return rc;
+    const double2 *b = (const double2 *)rhs_dev;
+    double2 *x = (double2 *)x_dev;
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+
+    double bnorm = norm2(s, b, n, h, part, rc);
+    if (rc) return rc;
+    int iters = 0;
+    if (!opts->use_x0) launch_zero(x, n, s);
+    if (bnorm == 0.0) {
+        launch_zero(x, n, s);
+        if (info) *info = {0, 1, 0.0, 0.0};
+        return IFS_OK;
+    }
+    // r = b - M x
+    auto residual = [&]() -> int {
+        int q = ifs_matvec_device(c, x, r, n);
+        if (q) return q;
+        ++iters;
+        launch_axpby(one, b, mone, r, n, s);
+        return IFS_OK;
+    };
+    if ((rc = residual())) return rc;
+    double beta = norm2(s, r, n, h, part, rc);
+    if (rc) return rc;
+    std::vector<cplx> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), hv, hv2;
+    int converged = beta / bnorm < opts->tol;
+    int breakdown = 0;
+    while (!converged && iters < opts->max_iter) {
+        launch_scale_copy(make_double2(1.0 / beta, 0.0), r, V, n, s);
+        std::fill(g.begin(), g.end(), cplx(0.0));
+        g[0] = beta;
+        int jend = 0;
+        for (int j = 0; j < m; ++j) {
+            double2 *vj = V + (int64_t)j * n;
+            if ((rc = ifs_matvec_device(c, vj, w, n))) return rc;
+            ++iters;
+            // CGS with one re-orthogonalisation pass
+            if ((rc = dots(s, V, n, j + 1, w, n, h, part, hv))) return rc;
+            if (hipMemcpyAsync(h, hv.data(), sizeof(double2) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+                return IFS_E_HIP;
+            launch_gemv_sub(V, n, j + 1, h, w, n, s);
+            if ((rc = dots(s, V, n, j + 1, w, n, h, part, hv2))) return rc;
+            if (hipMemcpyAsync(h, hv2.data(), sizeof(double2) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+                return IFS_E_HIP;
+            launch_gemv_sub(V, n, j + 1, h, w, n, s);
+            for (int i = 0; i <= j; ++i) hv[i] += hv2[i];
+            double hn = norm2(s, w, n, h, part, rc);
+            if (rc) return rc;
+            // Hessenberg column j, previous rotations, new rotation
+            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i];
+            H[(size_t)(j + 1) * m + j] = hn;
+            for (int i = 0; i < j; ++i) {
+                cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                H[(size_t)(i + 1) * m + j] = -std::conj(sn[i]) * a + cs[i] * bb;
+            }
+            cplx a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+            double ra = std::abs(a), rb = std::abs(bb), rr = std::hypot(ra, rb);
+            if (rr == 0.0) {
+                cs[j] = 1.0;
+                sn[j] = 0.0;
+            } else if (ra == 0.0) {
+                cs[j] = 0.0;
+                sn[j] = std::conj(bb) / rb;
+                H[(size_t)j * m + j] = rb;
+            } else {
+                cs[j] = ra / rr;
+                sn[j] = (a / ra) * std::conj(bb) / rr;
+                H[(size_t)j * m + j] = (a / ra) * rr;
+            }
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -std::conj(sn[j]) * g[j];
+            g[j] = cs[j] * g[j];
+            jend = j + 1;
+            double est = std::abs(g[j + 1]) / bnorm;
+            if (hn == 0.0) {
+                breakdown = 1;
+                break;
+            }
+            if (est < opts->tol || iters >= opts->max_iter) break;
+            launch_scale_copy(make_double2(1.0 / hn, 0.0), w, V + (int64_t)(j + 1) * n, n, s);
+        }
+        // y = H^-1 g (upper triangular), x += V y
+        std::vector<cplx> y(jend);
+        for (int i = jend - 1; i >= 0; --i) {
+            cplx acc = g[i];
+            for (int k = i + 1; k < jend; ++k) acc -= H[(size_t)i * m + k] * y[k];
+            y[i] = acc / H[(size_t)i * m + i];
+        }
+        if (hipMemcpyAsync(h, y.data(), sizeof(double2) * jend, hipMemcpyHostToDevice, s) != hipSuccess)
+            return IFS_E_HIP;
+        launch_gemv_add(V, n, jend, h, x, n, s);
+        // true residual in FP64
+        if ((rc = residual())) return rc;
+        beta = norm2(s, r, n, h, part, rc);
+        if (rc) return rc;
+        converged = beta / bnorm < opts->tol;
+        if (breakdown && !converged) break;
+        (void)zero;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return IFS_E_HIP;
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (info) {
+        info->iterations = iters;
+        info->converged = converged;
+        info->rel_residual = beta / bnorm;
+        info->seconds = secs;
+    }
+    if (converged) return IFS_OK;
+    return breakdown ? IFS_E_BREAKDOWN : IFS_E_NOCONV;
+}
+
+extern "C" int ifs_solve(ifs_ctx *c, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
+                         ifs_solve_info *info) {
+    if (!c || !rhs || !x || !opts || n != ifs_vector_length(c)) return IFS_E_ARG;
+    hipStream_t s = ctx_stream(c);
+    void *db = nullptr, *dx = nullptr;
+    if (hipMalloc(&db, 16ull * n) != hipSuccess) return IFS_E_NOMEM;
+    if (hipMalloc(&dx, 16ull * n) != hipSuccess) {
+        (void)hipFree(db);
+        return IFS_E_NOMEM;
+    }
+    int rc = IFS_OK;
+    if (hipMemcpyAsync(db, rhs, 16ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (opts->use_x0 && hipMemcpyAsync(dx, x, 16ull * n, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = IFS_E_HIP;
+    if (!rc) rc = ifs_solve_device(c, db, dx, n, opts, info);
+    if (rc == IFS_OK || rc == IFS_E_NOCONV || rc == IFS_E_BREAKDOWN) {
+        if (hipMemcpyAsync(x, dx, 16ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = IFS_E_HIP;
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    return rc;
+}
+
+// x-polarised plane wave along +z: p_{n,+-1}, q_{n,+-1} from the closed forms of
+// pi_mn, tau_mn at cos(alpha) = 1 (libmath/src/math/lib_math_legendre.f90:834-962,
+// Condon-Shortley phase removed for odd |m|) in
+// p = tau_mn e^{-i m beta} e^{i k.d} / (n(n+1)),  q = pi_mn e^{-i m beta} e^{i k.d} / (n(n+1))
+// (FoggySim/src/lib/mie_theory/lib_mie_illumination.f90:295-354, :636-763),
+// then rhs = (a_n p, b_n q) (lib_mie_ms_solver_interface.f90:230-315).
+extern "C" int ifs_plane_wave_rhs(ifs_ctx *c, double *rhs, int64_t n) {
+    if (!c || !rhs || n != ifs_vector_length(c)) return IFS_E_ARG;
+    const int ns = ctx_ns(c), N = ctx_N(c);
+    const double k = ctx_k(c);
+    const double *pos = ctx_pos(c);
+    std::vector<cplx> an((size_t)ns * N), bn((size_t)ns * N);
+    int rc = ifs_get_mie(c, (double *)an.data(), (double *)bn.data());
+    if (rc) return rc;
+    cplx *out = (cplx *)rhs;
+    const int64_t blk = (int64_t)N * (N + 2);
+    for (int64_t i = 0; i < n; ++i) out[i] = 0.0;
+    for (int j = 0; j < ns; ++j) {
+        cplx ph = std::exp(cplx(0.0, k * pos[3 * j + 2]));
+        for (int nn = 1; nn <= N; ++nn) {
+            double inv = 1.0 / (nn * (nn + 1.0));
+            double pi_p = nn * (nn + 1.0) / 2.0, pi_m = 0.5;
+            double tau_p = nn * (nn + 1.0) / 2.0, tau_m = -0.5;
+            cplx pp = ph * tau_p * inv, pm = ph * tau_m * inv;
+            cplx qp = ph * pi_p * inv, qm = ph * pi_m * inv;
+            int64_t o = (int64_t)j * 2 * blk;
+            int64_t ip = nn * nn - 1 + 1 + nn, im = nn * nn - 1 - 1 + nn;
+            out[o + ip] = an[(size_t)j * N + nn - 1] * pp;
+            out[o + im] = an[(size_t)j * N + nn - 1] * pm;
+            out[o + blk + ip] = bn[(size_t)j * N + nn - 1] * qp;
+            out[o + blk + im] = bn[(size_t)j * N + nn - 1] * qm;
+        }
+    }
+    return IFS_OK;
+}

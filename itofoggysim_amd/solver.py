@@ -1,0 +1,149 @@
+"""Host-side mirror of the reference's multi-sphere solver interface.
+
+``MultiSphereSolver`` follows lib_mie_ms_solver_interface
+(FoggySim/src/lib/mie_theory/lib_mie_ms_solver_interface.f90): the constructor
+builds the problem (:91-129), ``calculate_vector_b`` is the GMRES matvec callback
+(:739-783), ``get_vector_b`` the right-hand side (:230-315) and ``solve`` the
+GMRES run (lib_solver_GMRES.f90:116-316).  All compute happens in libifs.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+class MultiSphereSolver:
+    def __init__(self, centres, radii, m_rel, k, n_global=None, lmax=None, z_selector=3, ref_row_zeroing=False,
+                 use_fmm=False, leaf_size=32, fmm_d0=6.0, fmm_lmax_cap=0):
+        centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        ns = radii.shape[0]
+        m_rel = np.ascontiguousarray(np.broadcast_to(np.asarray(m_rel, dtype=np.complex128), (ns,)))
+        if lmax is None:
+            x = k * radii
+            lmax = np.ceil(x + 4.05 * np.cbrt(x) + 2.0).astype(np.int32)
+        lmax = np.ascontiguousarray(lmax, dtype=np.int32)
+        if n_global is None:
+            n_global = int(lmax.max())
+        self._keep = (centres, radii, m_rel, lmax)
+        cl = _lib.Cluster(ns, centres.ctypes.data, radii.ctypes.data, m_rel.ctypes.data, lmax.ctypes.data)
+        pr = _lib.Params()
+        pr.k = float(k)
+        pr.n_global = int(n_global)
+        pr.z_selector = int(z_selector)
+        pr.ref_row_zeroing = int(bool(ref_row_zeroing))
+        pr.use_fmm = int(bool(use_fmm))
+        pr.leaf_size = int(leaf_size)
+        pr.fmm_d0 = float(fmm_d0)
+        pr.fmm_lmax_cap = int(fmm_lmax_cap)
+        h = ctypes.c_void_p()
+        check(lib().ifs_create(ctypes.byref(h), ctypes.byref(cl), ctypes.byref(pr)), "ifs_create")
+        self._h = h
+        self.n_spheres = ns
+        self.n_global = int(n_global)
+        self.lmax = lmax
+        self.k = float(k)
+        self.n = int(lib().ifs_vector_length(h))
+
+    # -- reference interface ------------------------------------------------
+    def calculate_vector_b(self, vector_x):
+        """b = (I + T A) x on host arrays (lib_mie_ms_solver_calculate_vector_b)."""
+        x = np.ascontiguousarray(vector_x, dtype=np.complex128)
+        if x.shape != (self.n,):
+            raise ValueError(f"vector_x has shape {x.shape}, expected ({self.n},)")
+        b = np.empty_like(x)
+        check(lib().ifs_matvec(self._h, x.ctypes.data, b.ctypes.data, self.n), "ifs_matvec")
+        return b
+
+    def calculate_vector_b_device(self, x_ptr, b_ptr):
+        """Same on device pointers (e.g. torch tensors' data_ptr()), enqueued on the launch stream."""
+        check(lib().ifs_matvec_device(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(b_ptr), self.n),
+              "ifs_matvec_device")
+
+    def get_vector_b(self):
+        """Right-hand side for an x-polarised plane wave along +z (lib_mie_ms_solver_get_vector_b_all)."""
+        rhs = np.empty(self.n, dtype=np.complex128)
+        check(lib().ifs_plane_wave_rhs(self._h, rhs.ctypes.data, self.n), "ifs_plane_wave_rhs")
+        return rhs
+
+    def mie_coefficients(self):
+        """(a_n^j, b_n^j) as arrays [n_spheres, N] (lib_mie_type_func_get_sphere_parameter)."""
+        an = np.empty((self.n_spheres, self.n_global), dtype=np.complex128)
+        bn = np.empty_like(an)
+        check(lib().ifs_get_mie(self._h, an.ctypes.data, bn.ctypes.data), "ifs_get_mie")
+        return an, bn
+
+    def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True):
+        """Restarted GMRES on the device; returns (x, info dict) (lib_mie_ms_solver_run)."""
+        b = self.get_vector_b() if rhs is None else np.ascontiguousarray(rhs, dtype=np.complex128)
+        x = np.zeros(self.n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None))
+        inf = _lib.SolveInfo()
+        rc = lib().ifs_solve(self._h, b.ctypes.data, x.ctypes.data, self.n, ctypes.byref(op), ctypes.byref(inf))
+        if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
+            check(rc, "ifs_solve")
+        return x, dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
+                       seconds=inf.seconds)
+
+    def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False):
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)))
+        inf = _lib.SolveInfo()
+        rc = lib().ifs_solve_device(self._h, ctypes.c_void_p(rhs_ptr), ctypes.c_void_p(x_ptr), self.n,
+                                    ctypes.byref(op), ctypes.byref(inf))
+        if rc not in (0, -5, -6):
+            check(rc, "ifs_solve_device")
+        return dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
+                    seconds=inf.seconds, status=rc)
+
+    # -- instrumentation ------------------------------------------------------
+    def set_stream(self, stream_ptr):
+        check(lib().ifs_set_stream(self._h, ctypes.c_void_p(stream_ptr)), "ifs_set_stream")
+
+    def stream(self):
+        return lib().ifs_get_stream(self._h)
+
+    def profile(self, enable):
+        check(lib().ifs_profile_enable(self._h, int(bool(enable))), "ifs_profile_enable")
+
+    def profile_read(self, cap=64):
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_int64 * cap)()
+        n = lib().ifs_profile_read(self._h, ctypes.cast(names, ctypes.c_void_p), ctypes.cast(ms, ctypes.c_void_p),
+                                   ctypes.cast(cnt, ctypes.c_void_p), cap)
+        if n < 0:
+            check(n, "ifs_profile_read")
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(n)}
+
+    def matvec_cost(self):
+        f = ctypes.c_double()
+        b = ctypes.c_double()
+        check(lib().ifs_matvec_cost(self._h, ctypes.byref(f), ctypes.byref(b)), "ifs_matvec_cost")
+        return f.value, b.value
+
+    def fmm_stats(self):
+        counts = (ctypes.c_int64 * 6)()
+        lv = (ctypes.c_int32 * 32)()
+        check(lib().ifs_fmm_stats(self._h, ctypes.cast(counts, ctypes.c_void_p), ctypes.cast(lv, ctypes.c_void_p)),
+              "ifs_fmm_stats")
+        kinds = ["P2M", "M2M", "M2L", "L2L", "L2P", "P2P"]
+        return {k: counts[i] for i, k in enumerate(kinds)}, {l: lv[l] for l in range(32) if lv[l]}
+
+    def tree_export(self):
+        u = np.empty(self.n_spheres, dtype=np.int64)
+        lv = np.empty(2, dtype=np.int32)
+        check(lib().ifs_tree_export(self._h, u.ctypes.data, lv.ctypes.data), "ifs_tree_export")
+        return u, int(lv[0]), int(lv[1])
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ifs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
