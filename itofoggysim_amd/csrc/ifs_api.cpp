@@ -19,136 +19,16 @@
 #include <vector>
 
 #include "../../include/ifs.h"
+#include "ifs_engine.h"
 #include "ifs_fmm.h"
 #include "ifs_internal.h"
 
 using namespace ifs;
 
-namespace {
-
 #define HIP_OK(expr)                          \
     do {                                      \
         if ((expr) != hipSuccess) return IFS_E_HIP; \
     } while (0)
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t b) {
-        if (p && bytes >= b) return IFS_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (b == 0) return IFS_OK;
-        if (hipMalloc(&p, b) != hipSuccess) {
-            p = nullptr;
-            return IFS_E_NOMEM;
-        }
-        bytes = b;
-        return IFS_OK;
-    }
-    template <class T> T *as() const { return (T *)p; }
-};
-
-// Chunk of a translation batch that fits the workspace.
-struct Chunk {
-    int t_begin = 0, t_end = 0;      // task range (device task array)
-    int s_begin = 0, s_end = 0;      // segment range
-    int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;
-    int maxpairs = 0, maxK = 0, max_nmax = 0, max_Ls = 0;
-    int64_t max_rows_y = 0, max_rows_z = 0;
-    int rowback_blocks = 1;
-};
-
-// A translation batch: tasks from one source pool into one target pool.
-struct Batch {
-    std::vector<Task> tasks;     // sorted by target
-    std::vector<Segment> segs;
-    std::vector<Chunk> chunks;
-    DevBuf d_tasks, d_segs;
-    bool cached = false;         // tables generated once and kept
-    bool tables_ready = false;
-    std::string name;
-};
-
-struct Profiler {
-    bool on = false;
-    struct Rec {
-        const char *name;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    std::vector<hipEvent_t> pool;
-    std::map<std::string, std::pair<double, int64_t>> totals;
-    hipEvent_t get() {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        return e;
-    }
-    void flush() {
-        for (auto &r : recs) {
-            (void)hipEventSynchronize(r.b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, r.a, r.b);
-            auto &t = totals[r.name];
-            t.first += ms;
-            t.second += 1;
-            pool.push_back(r.a);
-            pool.push_back(r.b);
-        }
-        recs.clear();
-    }
-    ~Profiler() {
-        for (auto &r : recs) {
-            (void)hipEventDestroy(r.a);
-            (void)hipEventDestroy(r.b);
-        }
-        for (auto e : pool) (void)hipEventDestroy(e);
-    }
-};
-
-}  // namespace
-
-struct ifs_ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    ifs_params prm{};
-    int ns = 0;
-    int N = 0;
-    std::vector<double> pos;        // 3*ns
-    std::vector<double> radius;
-    std::vector<std::complex<double>> mrel;
-    std::vector<int32_t> Lj;        // requested per-sphere truncation
-    std::vector<int32_t> Leff;      // min(L_j, N)
-    std::vector<int32_t> active;    // row activity (reference row zeroing)
-    std::vector<int64_t> off;       // compact normalised offsets
-    int64_t ncompact = 0;           // complex entries of the compact vector
-    int64_t nflat = 0;              // complex entries of the flat vector
-
-    DevBuf d_Leff, d_active, d_off, d_snm, d_an, d_bn, d_x, d_b;
-    DevBuf d_xh, d_acc;             // compact normalised input / accumulator
-    DevBuf ws_d, ws_s, ws_ax, ws_y, ws_z;
-    size_t ws_budget = 0;
-
-    Batch direct;                   // sphere -> sphere (direct all pairs)
-    fmm::Fmm *fmm = nullptr;
-
-    // Krylov workspace
-    DevBuf k_V, k_w, k_r, k_part, k_h, k_xtmp;
-    int k_restart = 0;
-
-    Profiler prof;
-    double flops_mv = 0.0, bytes_mv = 0.0;
-};
 
 namespace {
 
@@ -169,7 +49,9 @@ std::vector<double> make_snm(int N) {
     return s;
 }
 
-Task make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_t src, int64_t dst) {
+}  // namespace
+
+Task ifs::make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_t src, int64_t dst) {
     Task T{};
     double r = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
     T.src = src;
@@ -183,8 +65,6 @@ Task make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_t src
     T.kind = kind;
     return T;
 }
-
-}  // namespace
 
 namespace ifs {
 
@@ -208,7 +88,7 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         sizes(T, nd, nsct, nax, ny, nz);
         tot_bytes += nd * 8 + (nsct + nax + ny + nz) * 16;
     }
-    B.cached = allow_cache && (size_t)tot_bytes <= budget_bytes;
+    B.cached = allow_cache && (size_t)tot_bytes + c->ws_cached <= budget_bytes;
     size_t chunk_budget = B.cached ? (size_t)tot_bytes + 1 : budget_bytes;
 
     Chunk cur;
@@ -278,11 +158,18 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         mz = std::max(mz, ch.n_z);
     }
     int rc;
-    if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, md * 8))) != IFS_OK) return rc;
-    if ((rc = c->ws_s.alloc(std::max<size_t>(c->ws_s.bytes, ms * 16))) != IFS_OK) return rc;
-    if ((rc = c->ws_ax.alloc(std::max<size_t>(c->ws_ax.bytes, ma * 16))) != IFS_OK) return rc;
-    if ((rc = c->ws_y.alloc(std::max<size_t>(c->ws_y.bytes, my * 16))) != IFS_OK) return rc;
-    if ((rc = c->ws_z.alloc(std::max<size_t>(c->ws_z.bytes, mz * 16))) != IFS_OK) return rc;
+    if (B.cached) {
+        if ((rc = B.own_d.alloc(md * 8)) || (rc = B.own_s.alloc(ms * 16)) || (rc = B.own_ax.alloc(ma * 16)) ||
+            (rc = B.own_y.alloc(my * 16)) || (rc = B.own_z.alloc(mz * 16)))
+            return rc;
+        c->ws_cached += (size_t)(md * 8 + (ms + ma + my + mz) * 16);
+    } else {
+        if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, md * 8))) != IFS_OK) return rc;
+        if ((rc = c->ws_s.alloc(std::max<size_t>(c->ws_s.bytes, ms * 16))) != IFS_OK) return rc;
+        if ((rc = c->ws_ax.alloc(std::max<size_t>(c->ws_ax.bytes, ma * 16))) != IFS_OK) return rc;
+        if ((rc = c->ws_y.alloc(std::max<size_t>(c->ws_y.bytes, my * 16))) != IFS_OK) return rc;
+        if ((rc = c->ws_z.alloc(std::max<size_t>(c->ws_z.bytes, mz * 16))) != IFS_OK) return rc;
+    }
     if ((rc = B.d_tasks.alloc(sizeof(Task) * std::max<size_t>(1, n))) != IFS_OK) return rc;
     if ((rc = B.d_segs.alloc(sizeof(Segment) * std::max<size_t>(1, B.segs.size()))) != IFS_OK) return rc;
     if (n) HIP_OK(hipMemcpy(B.d_tasks.p, B.tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice));
@@ -292,28 +179,27 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
     return IFS_OK;
 }
 
-// Profiling wrapper
-struct Scope {
-    ifs_ctx *c;
-    const char *name;
-    hipEvent_t a{}, b{};
-    Scope(ifs_ctx *c_, const char *n) : c(c_), name(n) {
-        if (c->prof.on) {
-            a = c->prof.get();
-            (void)hipEventRecord(a, c->stream);
-        }
+Scope::Scope(ifs_ctx *c_, const char *n) : c(c_), name(n) {
+    if (c->prof.on) {
+        a = c->prof.get();
+        (void)hipEventRecord(a, c->stream);
     }
-    ~Scope() {
-        if (c->prof.on) {
-            b = c->prof.get();
-            (void)hipEventRecord(b, c->stream);
-            c->prof.recs.push_back({name, a, b});
-        }
+}
+Scope::~Scope() {
+    if (c->prof.on) {
+        b = c->prof.get();
+        (void)hipEventRecord(b, c->stream);
+        c->prof.recs.push_back({name, a, b});
     }
-};
+}
 
 int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) {
     const Task *dt = B.d_tasks.as<Task>();
+    double *wd = B.cached ? B.own_d.as<double>() : c->ws_d.as<double>();
+    double2 *wsc = B.cached ? B.own_s.as<double2>() : c->ws_s.as<double2>();
+    double2 *wax = B.cached ? B.own_ax.as<double2>() : c->ws_ax.as<double2>();
+    double2 *wy = B.cached ? B.own_y.as<double2>() : c->ws_y.as<double2>();
+    double2 *wz = B.cached ? B.own_z.as<double2>() : c->ws_z.as<double2>();
     const Segment *dsg = B.d_segs.as<Segment>();
     for (auto &ch : B.chunks) {
         int nt = ch.t_end - ch.t_begin;
@@ -323,32 +209,28 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) 
         if (!(B.cached && B.tables_ready)) {
             {
                 Scope s(c, "gen_wigner");
-                launch_gen_wigner(tt, nt, ch.maxpairs, c->ws_d.as<double>(), c->stream);
+                launch_gen_wigner(tt, nt, ch.maxpairs, wd, c->stream);
             }
             {
                 Scope s(c, "gen_sect");
-                launch_gen_sect(tt, nt, ch.max_nmax, c->ws_s.as<double2>(), c->stream);
+                launch_gen_sect(tt, nt, ch.max_nmax, wsc, c->stream);
             }
             {
                 Scope s(c, "gen_axial");
-                launch_gen_axial(tt, nt, ch.maxK, ch.max_nmax, c->ws_s.as<double2>(), c->ws_ax.as<double2>(),
-                                 c->stream);
+                launch_gen_axial(tt, nt, ch.maxK, ch.max_nmax, wsc, wax, c->stream);
             }
         }
         {
             Scope s(c, "rot_fwd");
-            launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, src_pool, c->ws_d.as<double>(), c->ws_y.as<double2>(),
-                           c->stream);
+            launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, src_pool, wd, wy, c->stream);
         }
         {
             Scope s(c, "axial_apply");
-            launch_axial_apply(tt, nt, ch.max_rows_z, c->ws_ax.as<double2>(), c->ws_y.as<double2>(),
-                               c->ws_z.as<double2>(), c->stream);
+            launch_axial_apply(tt, nt, ch.max_rows_z, wax, wy, wz, c->stream);
         }
         {
             Scope s(c, "rot_back");
-            launch_rot_back(dt, dsg + ch.s_begin, nsg, ch.rowback_blocks, c->ws_d.as<double>(), c->ws_z.as<double2>(),
-                            dst_pool, c->stream);
+            launch_rot_back(dt, dsg + ch.s_begin, nsg, ch.rowback_blocks, wd, wz, dst_pool, c->stream);
         }
         if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
     }
@@ -372,6 +254,9 @@ int ctx_ns(ifs_ctx *c) { return c->ns; }
 int ctx_N(ifs_ctx *c) { return c->N; }
 const double *ctx_pos(ifs_ctx *c) { return c->pos.data(); }
 double ctx_k(ifs_ctx *c) { return c->prm.k; }
+
+double2 *c_bhat(ifs_ctx *c, int64_t n) { return c->k_bhat.alloc(16ull * n) ? nullptr : c->k_bhat.as<double2>(); }
+double2 *c_xhat(ifs_ctx *c, int64_t n) { return c->k_xhat.alloc(16ull * n) ? nullptr : c->k_xhat.as<double2>(); }
 
 int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **w, double2 **r, double2 **part,
                      double2 **h, int nblocks) {
@@ -481,7 +366,7 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     // workspace budget: a fraction of free memory
     size_t freeb = 0, totb = 0;
     (void)hipMemGetInfo(&freeb, &totb);
-    c->ws_budget = std::min<size_t>(freeb / 4, (size_t)48 << 30);
+    c->ws_budget = std::min<size_t>(freeb / 4, (size_t)16 << 30);
 
     if (!p->use_fmm) {
         // direct all-pairs plan: for each target j, sources l != j in ascending order
@@ -529,14 +414,17 @@ int ifs_set_stream(ifs_ctx *c, void *s) {
 
 void *ifs_get_stream(ifs_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
-int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
-    if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
-    const double2 *x = (const double2 *)x_dev;
-    double2 *b = (double2 *)b_dev;
+}  // extern "C"
+
+namespace ifs {
+// b = M x in the reference basis, or b_hat = S M S^-1 x_hat in the normalised
+// basis (normalised = 1), both in the flat layout.
+int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised) {
+    const double *snm = normalised ? nullptr : c->d_snm.as<double>();
     {
         Scope s(c, "pack_in");
-        launch_pack_in(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_off.as<int64_t>(), c->d_snm.as<double>(), x,
-                       c->d_xh.as<double2>(), 1, 0, 0, c->stream);
+        launch_pack_in(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_off.as<int64_t>(), snm, x, c->d_xh.as<double2>(),
+                       1, 0, 0, c->stream);
     }
     {
         Scope s(c, "zero");
@@ -552,11 +440,24 @@ int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
     {
         Scope s(c, "finalize");
         launch_finalize(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_active.as<int32_t>(), c->d_off.as<int64_t>(),
-                        c->d_snm.as<double>(), c->d_an.as<double2>(), c->d_bn.as<double2>(), x,
-                        c->d_acc.as<double2>(), b, 1, 0, 0, c->stream);
+                        snm, c->d_an.as<double2>(), c->d_bn.as<double2>(), x, c->d_acc.as<double2>(), b, 1, 0, 0,
+                        c->stream);
     }
     if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
     return IFS_OK;
+}
+
+int scale_flat(ifs_ctx *c, const double2 *x, double2 *y, int inverse) {
+    launch_scale_flat(c->ns, c->N, c->d_snm.as<double>(), x, y, inverse, c->stream);
+    return hipPeekAtLastError() == hipSuccess ? IFS_OK : IFS_E_HIP;
+}
+}  // namespace ifs
+
+extern "C" {
+
+int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
+    if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
+    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0);
 }
 
 int ifs_matvec(ifs_ctx *c, const double *x, double *b, int64_t n) {

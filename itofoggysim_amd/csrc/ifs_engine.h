@@ -1,0 +1,154 @@
+// Translation engine and context (internal): shared by ifs_api.cpp, ifs_fmm.cpp
+// and ifs_solver.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ifs.h"
+#include "ifs_internal.h"
+
+namespace ifs {
+namespace fmm {
+struct Fmm;
+}
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t b) {
+        if (p && bytes >= b) return IFS_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (b == 0) return IFS_OK;
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return IFS_E_NOMEM;
+        }
+        bytes = b;
+        return IFS_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// Chunk of a translation batch that fits the workspace.
+struct Chunk {
+    int t_begin = 0, t_end = 0;      // task range (device task array)
+    int s_begin = 0, s_end = 0;      // segment range
+    int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;
+    int maxpairs = 0, maxK = 0, max_nmax = 0, max_Ls = 0;
+    int64_t max_rows_y = 0, max_rows_z = 0;
+    int rowback_blocks = 1;
+};
+
+// A translation batch: tasks from one source pool into one target pool.
+struct Batch {
+    std::vector<Task> tasks;     // sorted by target
+    std::vector<Segment> segs;
+    std::vector<Chunk> chunks;
+    DevBuf d_tasks, d_segs;
+    DevBuf own_d, own_s, own_ax, own_y, own_z;  // workspace of a cached batch
+    bool cached = false;         // tables generated once and kept
+    bool tables_ready = false;
+    std::string name;
+};
+
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, int64_t>> totals;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void flush() {
+        for (auto &r : recs) {
+            (void)hipEventSynchronize(r.b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            auto &t = totals[r.name];
+            t.first += ms;
+            t.second += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    ~Profiler() {
+        for (auto &r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace ifs
+
+struct ifs_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    ifs_params prm{};
+    int ns = 0;
+    int N = 0;
+    std::vector<double> pos;        // 3*ns
+    std::vector<double> radius;
+    std::vector<std::complex<double>> mrel;
+    std::vector<int32_t> Lj;        // requested per-sphere truncation
+    std::vector<int32_t> Leff;      // min(L_j, N)
+    std::vector<int32_t> active;    // row activity (reference row zeroing)
+    std::vector<int64_t> off;       // compact normalised offsets
+    int64_t ncompact = 0;           // complex entries of the compact vector
+    int64_t nflat = 0;              // complex entries of the flat vector
+
+    ifs::DevBuf d_Leff, d_active, d_off, d_snm, d_an, d_bn, d_x, d_b;
+    ifs::DevBuf d_xh, d_acc;             // compact normalised input / accumulator
+    ifs::DevBuf ws_d, ws_s, ws_ax, ws_y, ws_z;
+    size_t ws_budget = 0;
+    size_t ws_cached = 0;           // bytes held by cached batches
+
+    ifs::Batch direct;              // sphere -> sphere (direct all pairs)
+    ifs::fmm::Fmm *fmm = nullptr;
+
+    // Krylov workspace
+    ifs::DevBuf k_V, k_w, k_r, k_part, k_h, k_bhat, k_xhat;
+    int k_restart = 0;
+
+    ifs::Profiler prof;
+    double flops_mv = 0.0, bytes_mv = 0.0;
+};
+
+
+namespace ifs {
+int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache);
+int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool);
+void batch_cost(const Batch &B, double &flops, double &bytes);
+Task make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_t src, int64_t dst);
+
+// Profiling scope: records HIP events around the launches it encloses.
+struct Scope {
+    ifs_ctx *c;
+    const char *name;
+    hipEvent_t a{}, b{};
+    Scope(ifs_ctx *c_, const char *n);
+    ~Scope();
+};
+}  // namespace ifs

@@ -1,17 +1,274 @@
-// MLFMM (placeholder until the octree and passes land).
+// MLFMM plan and passes.
+//
+// Restates libmlfmm's algorithm for the Mie plug-in
+// (libmlfmm/src/lib/lib_ml_fmm.f90:231-262 and the callbacks of
+// FoggySim/src/lib/mie_theory/lib_mie_ms_ml_fmm_interface.f90), with one
+// translation convention throughout: every translation uses
+// d = origin(source) - origin(target) in Xu's coefficients, regular waves (j_n)
+// for outgoing->outgoing (P2M, M2M) and regular->regular (L2L, L2P), and the
+// sphere-interaction kind (h1_n) for outgoing->regular (M2L, near field).
+//
+//   upward   : C(leaf) = sum P2M(sphere)              (lib_ml_fmm.f90:387-508)
+//              C(parent) = sum M2M(C(child))          (:520-640)
+//   downward : D(box) = sum_{I4(box)} M2L(C(src))     (:691-847)
+//              D(box) += L2L(D(parent))               (:861-982)
+//   final    : v_j = L2P(D(leaf)) + sum_{E2, i != j} P2P(x_i)  (:994-1204)
+// Boxes carry a per-level truncation L = kD + 1.8 d0^(2/3) (kD)^(1/3), D the box edge.
 #include "ifs_fmm.h"
-#include "../../include/ifs.h"
+
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+#include <vector>
+
+#include "ifs_engine.h"
+#include "ifs_tree.h"
 
 namespace ifs {
 namespace fmm {
-struct Fmm {};
-int build(ifs_ctx *, Fmm **out) {
-    *out = nullptr;
-    return IFS_E_ARG;
+
+struct Level {
+    int l = 0;
+    int L = 0;
+    double edge = 0.0;
+    std::vector<int64_t> boxes;                 // non-empty boxes, ascending Morton index
+    std::unordered_map<int64_t, int> index;     // Morton -> position
+    std::vector<double> centre;                 // 3 per box (unscaled)
+    int64_t off = 0;                            // pool offset of the first box (complex)
+    int64_t per_box() const { return 2LL * L * (L + 2); }
+};
+
+struct Fmm {
+    int l_min = 2, l_max = 2;
+    std::vector<Level> lv;                      // index l (0..l_max), only l_min..l_max used
+    std::vector<int64_t> leaf_of;               // sphere -> leaf Morton index
+    DevBuf C, D;                                // multipole / local pools
+    int64_t nC = 0;
+    Batch p2m, l2p, p2p;
+    std::vector<Batch> m2m, m2l, l2l;           // indexed by level
+    int64_t counts[6] = {0, 0, 0, 0, 0, 0};
+};
+
+static int box_L(double k, double edge, double d0, int cap) {
+    double kD = k * edge;
+    int L = (int)std::ceil(kD + 1.8 * std::pow(d0, 2.0 / 3.0) * std::cbrt(kD));
+    if (L < 1) L = 1;
+    if (cap > 0 && L > cap) L = cap;
+    return L;
 }
+
+int build(ifs_ctx *c, Fmm **out) {
+    *out = nullptr;
+    Fmm *f = new Fmm();
+    const int ns = c->ns;
+    const double k = c->prm.k;
+    const int s = c->prm.leaf_size > 0 ? c->prm.leaf_size : 10;
+    const double d0 = c->prm.fmm_d0 > 0.0 ? c->prm.fmm_d0 : 6.0;
+
+    // octree (lib_tree_constructor, libtree/src/lib/lib_tree.f90:147-270)
+    std::vector<double> scaled;
+    tree::Scaling S = tree::scale_points(c->pos.data(), ns, scaled);
+    std::vector<int64_t> u(ns);
+    for (int i = 0; i < ns; ++i) u[i] = tree::universal_index(&scaled[3 * i], tree::kThresholdLevel);
+    std::vector<int64_t> us = u;
+    std::sort(us.begin(), us.end());
+    f->l_min = tree::level_min(1);
+    f->l_max = std::max(tree::level_max(us, s), f->l_min);
+    f->l_max = std::min(f->l_max, tree::kThresholdLevel);
+    const int lmax = f->l_max, lmin = f->l_min;
+    f->leaf_of.resize(ns);
+    for (int i = 0; i < ns; ++i) f->leaf_of[i] = u[i] >> (3 * (tree::kThresholdLevel - lmax));
+
+    // hierarchy of non-empty boxes per level (lib_ml_fmm_hf_create_hierarchy,
+    // libmlfmm/src/lib/lib_ml_fmm_helper_functions.f90:366-446)
+    f->lv.resize(lmax + 1);
+    int64_t pool = 0;
+    for (int l = lmin; l <= lmax; ++l) {
+        Level &L = f->lv[l];
+        L.l = l;
+        L.edge = S.D / std::ldexp(1.0, l);
+        L.L = box_L(k, L.edge, d0, c->prm.fmm_lmax_cap);
+        std::vector<int64_t> b(ns);
+        for (int i = 0; i < ns; ++i) b[i] = f->leaf_of[i] >> (3 * (lmax - l));
+        std::sort(b.begin(), b.end());
+        b.erase(std::unique(b.begin(), b.end()), b.end());
+        L.boxes = b;
+        L.centre.resize(3 * b.size());
+        for (size_t q = 0; q < b.size(); ++q) {
+            L.index[b[q]] = (int)q;
+            double cs[3];
+            tree::centre(b[q], l, cs);
+            for (int d = 0; d < 3; ++d) L.centre[3 * q + d] = cs[d] * S.D + S.x_min[d];  // lib_tree.f90:376-389
+        }
+        L.off = pool;
+        pool += (int64_t)b.size() * L.per_box();
+    }
+    f->nC = pool;
+    int rc;
+    if ((rc = f->C.alloc(16ull * pool)) || (rc = f->D.alloc(16ull * pool))) {
+        delete f;
+        return rc;
+    }
+    auto dvec = [](const double *src, const double *dst, double *dv) {
+        for (int d = 0; d < 3; ++d) dv[d] = src[d] - dst[d];
+    };
+    const Level &LL = f->lv[lmax];
+
+    // P2M: sphere -> leaf box, grouped by target box (ascending sphere index)
+    {
+        std::vector<std::vector<int>> members(LL.boxes.size());
+        for (int i = 0; i < ns; ++i) members[LL.index.at(f->leaf_of[i])].push_back(i);
+        for (size_t q = 0; q < LL.boxes.size(); ++q)
+            for (int i : members[q]) {
+                double dv[3];
+                dvec(&c->pos[3 * i], &LL.centre[3 * q], dv);
+                f->p2m.tasks.push_back(make_task(dv, k, 1, c->Leff[i], LL.L, c->off[i], LL.off + q * LL.per_box()));
+            }
+        f->counts[0] = (int64_t)f->p2m.tasks.size();
+    }
+    // M2M: child (level l+1) -> parent (level l)
+    f->m2m.resize(lmax + 1);
+    for (int l = lmax - 1; l >= lmin; --l) {
+        const Level &P = f->lv[l], &Ch = f->lv[l + 1];
+        std::vector<std::vector<int>> kids(P.boxes.size());
+        for (size_t q = 0; q < Ch.boxes.size(); ++q) kids[P.index.at(tree::parent(Ch.boxes[q]))].push_back((int)q);
+        for (size_t p = 0; p < P.boxes.size(); ++p)
+            for (int q : kids[p]) {
+                double dv[3];
+                dvec(&Ch.centre[3 * q], &P.centre[3 * p], dv);
+                f->m2m[l].tasks.push_back(
+                    make_task(dv, k, 1, Ch.L, P.L, Ch.off + q * Ch.per_box(), P.off + p * P.per_box()));
+            }
+        f->counts[1] += (int64_t)f->m2m[l].tasks.size();
+    }
+    // M2L: I4 of every box, sources ascending
+    f->m2l.resize(lmax + 1);
+    for (int l = lmin; l <= lmax; ++l) {
+        const Level &V = f->lv[l];
+        for (size_t q = 0; q < V.boxes.size(); ++q) {
+            std::vector<int64_t> i4 = tree::domain_i4(V.boxes[q], l, 1);
+            for (int64_t src : i4) {
+                auto it = V.index.find(src);
+                if (it == V.index.end()) continue;
+                int p = it->second;
+                double dv[3];
+                dvec(&V.centre[3 * p], &V.centre[3 * q], dv);
+                f->m2l[l].tasks.push_back(
+                    make_task(dv, k, c->prm.z_selector, V.L, V.L, V.off + p * V.per_box(), V.off + q * V.per_box()));
+            }
+        }
+        f->counts[2] += (int64_t)f->m2l[l].tasks.size();
+    }
+    // L2L: parent (level l-1) -> child (level l)
+    f->l2l.resize(lmax + 1);
+    for (int l = lmin + 1; l <= lmax; ++l) {
+        const Level &P = f->lv[l - 1], &Ch = f->lv[l];
+        for (size_t q = 0; q < Ch.boxes.size(); ++q) {
+            int p = P.index.at(tree::parent(Ch.boxes[q]));
+            double dv[3];
+            dvec(&P.centre[3 * p], &Ch.centre[3 * q], dv);
+            f->l2l[l].tasks.push_back(
+                make_task(dv, k, 1, P.L, Ch.L, P.off + p * P.per_box(), Ch.off + q * Ch.per_box()));
+        }
+        f->counts[3] += (int64_t)f->l2l[l].tasks.size();
+    }
+    // L2P: leaf box -> sphere (targets ascending)
+    for (int i = 0; i < ns; ++i) {
+        int q = LL.index.at(f->leaf_of[i]);
+        double dv[3];
+        dvec(&LL.centre[3 * q], &c->pos[3 * i], dv);
+        f->l2p.tasks.push_back(make_task(dv, k, 1, LL.L, c->Leff[i], LL.off + q * LL.per_box(), c->off[i]));
+    }
+    f->counts[4] = (int64_t)f->l2p.tasks.size();
+    // P2P: spheres of E2 = I2(leaf box), i != j; sources ascending
+    {
+        std::vector<std::vector<int>> members(LL.boxes.size());
+        for (int i = 0; i < ns; ++i) members[LL.index.at(f->leaf_of[i])].push_back(i);
+        for (int j = 0; j < ns; ++j) {
+            if (!c->active[j]) continue;
+            std::vector<int64_t> i2 = tree::domain_i2(f->leaf_of[j], lmax, 1);
+            std::vector<int> src;
+            for (int64_t b : i2) {
+                auto it = LL.index.find(b);
+                if (it == LL.index.end()) continue;
+                for (int i : members[it->second])
+                    if (i != j) src.push_back(i);
+            }
+            std::sort(src.begin(), src.end());
+            for (int i : src) {
+                double dv[3];
+                dvec(&c->pos[3 * i], &c->pos[3 * j], dv);
+                f->p2p.tasks.push_back(
+                    make_task(dv, k, c->prm.z_selector, c->Leff[i], c->Leff[j], c->off[i], c->off[j]));
+            }
+        }
+        f->counts[5] = (int64_t)f->p2p.tasks.size();
+    }
+    // rows of inactive spheres (reference row zeroing) get no far field either
+    if (c->prm.ref_row_zeroing) {
+        std::vector<Task> keep;
+        for (auto &T : f->l2p.tasks) {
+            bool act = false;
+            for (int i = 0; i < ns; ++i)
+                if (c->off[i] == T.dst) act = c->active[i];
+            if (act) keep.push_back(T);
+        }
+        f->l2p.tasks.swap(keep);
+    }
+
+    auto fin = [&](Batch &B, const char *name) -> int {
+        B.name = name;
+        int r = batch_finalize(c, B, c->ws_budget, true);
+        if (!r) batch_cost(B, c->flops_mv, c->bytes_mv);
+        return r;
+    };
+    if ((rc = fin(f->p2m, "p2m"))) goto fail;
+    for (int l = lmin; l <= lmax; ++l) {
+        if (l < lmax && (rc = fin(f->m2m[l], "m2m"))) goto fail;
+        if ((rc = fin(f->m2l[l], "m2l"))) goto fail;
+        if (l > lmin && (rc = fin(f->l2l[l], "l2l"))) goto fail;
+    }
+    if ((rc = fin(f->l2p, "l2p"))) goto fail;
+    if ((rc = fin(f->p2p, "p2p"))) goto fail;
+    *out = f;
+    return IFS_OK;
+fail:
+    delete f;
+    return rc;
+}
+
 void destroy(Fmm *f) { delete f; }
-int run(ifs_ctx *, Fmm *, const double2 *, double2 *) { return IFS_E_ARG; }
-int tree_export(Fmm *, int64_t *, int32_t *) { return IFS_E_ARG; }
-int stats(Fmm *, int64_t *, int32_t *) { return IFS_E_ARG; }
+
+int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc) {
+    double2 *C = f->C.as<double2>(), *D = f->D.as<double2>();
+    launch_zero(C, f->nC, c->stream);
+    launch_zero(D, f->nC, c->stream);
+    int rc;
+    if ((rc = batch_run(c, f->p2m, xh, C))) return rc;
+    for (int l = f->l_max - 1; l >= f->l_min; --l)
+        if ((rc = batch_run(c, f->m2m[l], C, C))) return rc;
+    for (int l = f->l_min; l <= f->l_max; ++l) {
+        if ((rc = batch_run(c, f->m2l[l], C, D))) return rc;
+        if (l > f->l_min && (rc = batch_run(c, f->l2l[l], D, D))) return rc;
+    }
+    if ((rc = batch_run(c, f->l2p, D, acc))) return rc;
+    if ((rc = batch_run(c, f->p2p, xh, acc))) return rc;
+    return IFS_OK;
+}
+
+int tree_export(Fmm *f, int64_t *uindex_out, int32_t *levels_out) {
+    for (size_t i = 0; i < f->leaf_of.size(); ++i) uindex_out[i] = f->leaf_of[i];
+    levels_out[0] = f->l_min;
+    levels_out[1] = f->l_max;
+    return IFS_OK;
+}
+
+int stats(Fmm *f, int64_t counts[6], int32_t levels_L[32]) {
+    for (int i = 0; i < 6; ++i) counts[i] = f->counts[i];
+    for (int l = 0; l < 32; ++l) levels_L[l] = 0;
+    for (int l = f->l_min; l <= f->l_max && l < 32; ++l) levels_L[l] = f->lv[l].L;
+    return IFS_OK;
+}
+
 }  // namespace fmm
 }  // namespace ifs

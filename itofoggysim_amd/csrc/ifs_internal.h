@@ -106,6 +106,7 @@ void launch_finalize(int ns, int N, const int32_t *Leff, const int32_t *active, 
                      const double *snm, const double2 *an, const double2 *bn, const double2 *x, const double2 *acc,
                      double2 *b, int nrhs, int64_t ldx, int64_t ldacc, hipStream_t s);
 void launch_zero(double2 *p, int64_t n, hipStream_t s);
+void launch_scale_flat(int ns, int N, const double *snm, const double2 *x, double2 *y, int inverse, hipStream_t s);
 
 // BLAS-1 style kernels for the Krylov solver (FP64 complex).
 void launch_dots(const double2 *V, int64_t ldv, int nv, const double2 *w, int64_t n, double2 *out, double2 *partial,

@@ -552,7 +552,7 @@ __global__ void k_pack_in(int N, const int32_t *__restrict__ Leff, const int64_t
     const double2 *xs = x + r * ldx + (int64_t)j * 2 * blk;
     double2 *xd = xh + r * ldxh + off[j];
     for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < cnt; e += blockDim.x * gridDim.y) {
-        double s = snm[e];
+        double s = snm ? snm[e] : 1.0;
         xd[e] = cscale(s, xs[e]);
         xd[e + cnt] = cscale(s, xs[e + blk]);
     }
@@ -586,7 +586,7 @@ __global__ void k_finalize(int N, const int32_t *__restrict__ Leff, const int32_
             int n = (int)sqrt((double)(e + 1));
             while ((n + 1) * (n + 1) - 1 <= e) ++n;
             while (n * n - 1 > e) --n;
-            double inv = 1.0 / snm[e];
+            double inv = snm ? 1.0 / snm[e] : 1.0;
             va = cadd(va, cmul(an[(int64_t)j * N + n - 1], cscale(inv, as[e])));
             vb = cadd(vb, cmul(bn[(int64_t)j * N + n - 1], cscale(inv, as[e + cnt])));
         }
@@ -602,6 +602,26 @@ void launch_finalize(int ns, int N, const int32_t *Leff, const int32_t *active, 
     int yb = (N * (N + 2) + 255) / 256;
     hipLaunchKernelGGL(k_finalize, dim3(ns, yb, nrhs), dim3(256), 0, s, N, Leff, active, off, snm, an, bn, x, acc,
                        b, ldx, ldacc);
+}
+
+// y = s * x (or x / s) per flat block entry, all n <= N (reference layout)
+__global__ void k_scale_flat(int N, const double *__restrict__ snm, const double2 *__restrict__ x,
+                             double2 *__restrict__ y, int inverse) {
+    const int j = blockIdx.x;
+    const int blk = N * (N + 2);
+    const double2 *xs = x + (int64_t)j * 2 * blk;
+    double2 *ys = y + (int64_t)j * 2 * blk;
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < blk; e += blockDim.x * gridDim.y) {
+        double s = inverse ? 1.0 / snm[e] : snm[e];
+        ys[e] = cscale(s, xs[e]);
+        ys[e + blk] = cscale(s, xs[e + blk]);
+    }
+}
+
+void launch_scale_flat(int ns, int N, const double *snm, const double2 *x, double2 *y, int inverse, hipStream_t s) {
+    if (ns <= 0) return;
+    int yb = (N * (N + 2) + 255) / 256;
+    hipLaunchKernelGGL(k_scale_flat, dim3(ns, yb), dim3(256), 0, s, N, snm, x, y, inverse);
 }
 
 __global__ void k_zero(double2 *__restrict__ p, int64_t n) {

@@ -6,6 +6,12 @@
 // Here the Krylov basis lives in HBM; orthogonalisation is classical Gram-Schmidt
 // with one re-orthogonalisation pass (batched dot products, fixed reduction
 // order), and the small Hessenberg / Givens update runs on the host in FP64.
+//
+// The iteration runs in the normalised basis x_hat = S x, s(n,m) =
+// sqrt(n(n+1)(2n+1)(n-m)!/(n+m)!), where the operator is S (I + T A) S^-1.  In
+// Xu's basis the entries of A span ~1e+-96 at n ~ 20 (factorial ratios), and
+// Krylov iterations under the plain 2-norm stall; the tolerance therefore refers
+// to the relative residual in the normalised basis.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -24,10 +30,14 @@ namespace ifs {
 hipStream_t ctx_stream(ifs_ctx *c);
 int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **w, double2 **r, double2 **part,
                      double2 **h, int nblocks);
+double2 *c_bhat(ifs_ctx *c, int64_t n);
+double2 *c_xhat(ifs_ctx *c, int64_t n);
 int ctx_ns(ifs_ctx *c);
 int ctx_N(ifs_ctx *c);
 const double *ctx_pos(ifs_ctx *c);
 double ctx_k(ifs_ctx *c);
+int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised);
+int scale_flat(ifs_ctx *c, const double2 *x, double2 *y, int inverse);
 }  // namespace ifs
 
 namespace {
@@ -62,8 +72,13 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     double2 *V, *w, *r, *part, *h;
     int rc = ctx_krylov_alloc(c, m, n, &V, &w, &r, &part, &h, kDotBlocks);
     if (rc) return rc;
-    const double2 *b = (const double2 *)rhs_dev;
-    double2 *x = (double2 *)x_dev;
+    const double2 *b_in = (const double2 *)rhs_dev;
+    double2 *x_out = (double2 *)x_dev;
+    // normalised right-hand side and iterate live in the Krylov workspace
+    double2 *b = c_bhat(c, n), *x = c_xhat(c, n);
+    if (!b || !x) return IFS_E_NOMEM;
+    if ((rc = scale_flat(c, b_in, b, 0))) return rc;
+    if (opts->use_x0 && (rc = scale_flat(c, x_out, x, 0))) return rc;
     const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
 
     double bnorm = norm2(s, b, n, h, part, rc);
@@ -71,13 +86,13 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     int iters = 0;
     if (!opts->use_x0) launch_zero(x, n, s);
     if (bnorm == 0.0) {
-        launch_zero(x, n, s);
+        launch_zero(x_out, n, s);
         if (info) *info = {0, 1, 0.0, 0.0};
         return IFS_OK;
     }
     // r = b - M x
     auto residual = [&]() -> int {
-        int q = ifs_matvec_device(c, x, r, n);
+        int q = matvec_impl(c, x, r, 1);
         if (q) return q;
         ++iters;
         launch_axpby(one, b, mone, r, n, s);
@@ -96,7 +111,7 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         int jend = 0;
         for (int j = 0; j < m; ++j) {
             double2 *vj = V + (int64_t)j * n;
-            if ((rc = ifs_matvec_device(c, vj, w, n))) return rc;
+            if ((rc = matvec_impl(c, vj, w, 1))) return rc;
             ++iters;
             // CGS with one re-orthogonalisation pass
             if ((rc = dots(s, V, n, j + 1, w, n, h, part, hv))) return rc;
@@ -162,6 +177,7 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         if (breakdown && !converged) break;
         (void)zero;
     }
+    if ((rc = scale_flat(c, x, x_out, 1))) return rc;
     if (hipStreamSynchronize(s) != hipSuccess) return IFS_E_HIP;
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (info) {

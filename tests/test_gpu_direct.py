@@ -1,5 +1,7 @@
 """GPU parity of the direct (all-pairs) matvec, the Mie T-matrix and the solve,
 against the CPU oracle on the same seeded inputs."""
+import math
+
 import numpy as np
 import pytest
 
@@ -86,7 +88,12 @@ def test_solve_config1(oracle):
     assert info["converged"]
     an, bn = s.mie_coefficients()
     Mx = oracle.matvec_direct(c["centres"], c["lmax"], s.n_global, an, bn, c["k"], x, zsel=3)
-    res = np.linalg.norm(Mx - rhs) / np.linalg.norm(rhs)
-    print("oracle residual", res)
+    # residual in the normalised basis s(n,m) = sqrt(n(n+1)(2n+1)(n-m)!/(n+m)!)
+    N = s.n_global
+    snm = np.array([np.sqrt(n * (n + 1) * (2 * n + 1) * math.factorial(n - m) / math.factorial(n + m))
+                    for n in range(1, N + 1) for m in range(-n, n + 1)])
+    w = np.tile(np.concatenate([snm, snm]), s.n_spheres)
+    res = np.linalg.norm(w * (Mx - rhs)) / np.linalg.norm(w * rhs)
+    print("oracle residual (normalised basis)", res)
     assert res < 1e-7
     s.close()
