@@ -238,6 +238,98 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) 
     return IFS_OK;
 }
 
+int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
+    const size_t nc = B.cls.size();
+    B.regen_d.assign(nc, 1);
+    B.regen_ax.assign(nc, 1);
+    for (size_t i = 0; i < nc; ++i) {
+        const Task &T = B.gen[i];
+        int L = std::max(T.Ls, T.Lt);
+        int nmax = T.Lt + T.Ls + 2;
+        B.n_d = std::max<int64_t>(B.n_d, wig_size(L, T.K));
+        B.n_s = std::max<int64_t>(B.n_s, sect_size(T.K, nmax));
+        B.n_ax = std::max<int64_t>(B.n_ax, ax_size(T.Lt, T.Ls, T.K));
+        B.n_y = std::max<int64_t>(B.n_y, rows_size(T.Ls, T.K) * 4 * (int64_t)B.cls[i].ncols);
+        B.n_z = std::max<int64_t>(B.n_z, rows_size(T.Lt, T.K) * 4 * (int64_t)B.cls[i].ncols);
+        if (i > 0) {
+            const Task &P = B.gen[i - 1];
+            if (P.cth == T.cth && P.Ls == T.Ls && P.Lt == T.Lt && P.K == T.K) B.regen_d[i] = 0;
+            if (P.t == T.t && P.kind == T.kind && P.Ls == T.Ls && P.Lt == T.Lt && P.K == T.K) B.regen_ax[i] = 0;
+        }
+    }
+    int rc;
+    if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, B.n_d * 8))) ||
+        (rc = c->ws_s.alloc(std::max<size_t>(c->ws_s.bytes, B.n_s * 16))) ||
+        (rc = c->ws_ax.alloc(std::max<size_t>(c->ws_ax.bytes, B.n_ax * 16))) ||
+        (rc = c->ws_y.alloc(std::max<size_t>(c->ws_y.bytes, B.n_y * 8))) ||
+        (rc = c->ws_z.alloc(std::max<size_t>(c->ws_z.bytes, B.n_z * 8))))
+        return rc;
+    if ((rc = B.d_cls.alloc(sizeof(ClassDesc) * std::max<size_t>(1, nc))) ||
+        (rc = B.d_gen.alloc(sizeof(Task) * std::max<size_t>(1, nc))) ||
+        (rc = B.d_src.alloc(8 * std::max<size_t>(1, B.src.size()))) ||
+        (rc = B.d_dst.alloc(8 * std::max<size_t>(1, B.dst.size()))))
+        return rc;
+    if (nc) {
+        HIP_OK(hipMemcpy(B.d_cls.p, B.cls.data(), sizeof(ClassDesc) * nc, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(B.d_gen.p, B.gen.data(), sizeof(Task) * nc, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(B.d_src.p, B.src.data(), 8 * B.src.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(B.d_dst.p, B.dst.data(), 8 * B.dst.size(), hipMemcpyHostToDevice));
+    }
+    return IFS_OK;
+}
+
+int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool) {
+    const ClassDesc *dc = B.d_cls.as<ClassDesc>();
+    const Task *dg = B.d_gen.as<Task>();
+    double *wd = c->ws_d.as<double>();
+    double2 *wsc = c->ws_s.as<double2>();
+    double2 *wax = c->ws_ax.as<double2>();
+    double *wy = c->ws_y.as<double>();
+    double *wz = c->ws_z.as<double>();
+    for (size_t i = 0; i < B.cls.size(); ++i) {
+        const Task &T = B.gen[i];
+        if (B.regen_d[i]) {
+            Scope s(c, "class_gen_wigner");
+            int L = std::max(T.Ls, T.Lt);
+            launch_gen_wigner(dg + i, 1, (2 * T.K + 1) * (2 * L + 1), wd, c->stream);
+        }
+        if (B.regen_ax[i]) {
+            Scope s(c, "class_gen_axial");
+            launch_gen_sect(dg + i, 1, T.Lt + T.Ls + 2, wsc, c->stream);
+            launch_gen_axial(dg + i, 1, T.K, T.Lt + T.Ls + 2, wsc, wax, c->stream);
+        }
+        {
+            Scope s(c, "class_rot_fwd");
+            launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
+                         wy, wz, 0, c->stream);
+        }
+        {
+            Scope s(c, "class_axial");
+            launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
+                         wy, wz, 1, c->stream);
+        }
+        {
+            Scope s(c, "class_rot_back");
+            launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
+                         wy, wz, 2, c->stream);
+        }
+    }
+    if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
+    return IFS_OK;
+}
+
+void class_batch_cost(const ClassBatch &B, double &flops, double &bytes) {
+    for (size_t i = 0; i < B.cls.size(); ++i) {
+        const Task &T = B.gen[i];
+        double f = 0.0;
+        for (int n = 1; n <= T.Ls; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+        for (int n = 1; n <= T.Lt; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+        for (int k = -T.K; k <= T.K; ++k) f += 32.0 * (double)ax_block(k, T.Lt, T.Ls);
+        flops += f * B.cls[i].ncols;
+        bytes += (4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5) * B.cls[i].ncols;
+    }
+}
+
 void batch_cost(const Batch &B, double &flops, double &bytes) {
     for (auto &T : B.tasks) {
         double f = 0.0;

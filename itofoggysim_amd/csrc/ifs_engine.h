@@ -138,6 +138,21 @@ struct ifs_ctx {
 
 
 namespace ifs {
+// Translations grouped in classes that share one translation vector; run as
+// tiled products (ifs_gemm.hip) with one Wigner-d / coaxial table per class.
+struct ClassBatch {
+    std::vector<ClassDesc> cls;
+    std::vector<Task> gen;             // per class: table parameters (offsets 0)
+    std::vector<int64_t> src, dst;     // columns of all classes
+    std::vector<char> regen_d, regen_ax;
+    DevBuf d_cls, d_gen, d_src, d_dst;
+    int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;  // workspace (doubles for d, y, z; complex for s, ax)
+    std::string name;
+};
+int class_batch_finalize(ifs_ctx *c, ClassBatch &B);
+int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool);
+void class_batch_cost(const ClassBatch &B, double &flops, double &bytes);
+
 int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache);
 int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool);
 void batch_cost(const Batch &B, double &flops, double &bytes);

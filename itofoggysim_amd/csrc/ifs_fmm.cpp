@@ -17,7 +17,9 @@
 #include "ifs_fmm.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -45,7 +47,7 @@ struct Fmm {
     DevBuf C, D;                                // multipole / local pools
     int64_t nC = 0;
     Batch p2m, l2p, p2p;
-    std::vector<Batch> m2m, m2l, l2l;           // indexed by level
+    std::vector<ClassBatch> m2m, m2l, l2l;      // indexed by level
     int64_t counts[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -55,6 +57,39 @@ static int box_L(double k, double edge, double d0, int cap) {
     if (L < 1) L = 1;
     if (cap > 0 && L > cap) L = cap;
     return L;
+}
+
+// Appends one class (shared translation vector dv) with its columns.
+static void add_class(ClassBatch &B, const double *dv, double k, int kind, int Ls, int Lt,
+                      const std::vector<std::pair<int64_t, int64_t>> &cols) {
+    if (cols.empty()) return;
+    Task T = make_task(dv, k, kind, Ls, Lt, 0, 0);
+    T.off_d = T.off_s = T.off_ax = T.off_y = T.off_z = 0;
+    ClassDesc C{};
+    C.off_d = 0;
+    C.off_ax = 0;
+    C.phi = T.phi;
+    C.Ls = Ls;
+    C.Lt = Lt;
+    C.K = T.K;
+    C.ncols = (int)cols.size();
+    C.col_begin = (int)B.src.size();
+    for (auto &pr : cols) {
+        B.src.push_back(pr.first);
+        B.dst.push_back(pr.second);
+    }
+    B.cls.push_back(C);
+    B.gen.push_back(T);
+}
+
+static void box_coords(int64_t n, int l, int64_t v[3]) {
+    v[0] = v[1] = v[2] = 0;
+    for (int i = l - 1; i >= 0; --i) {
+        int64_t t = (n >> (3 * i)) & 7;
+        v[0] = (v[0] << 1) | ((t >> 2) & 1);
+        v[1] = (v[1] << 1) | ((t >> 1) & 1);
+        v[2] = (v[2] << 1) | (t & 1);
+    }
 }
 
 int build(ifs_ctx *c, Fmm **out) {
@@ -126,51 +161,75 @@ int build(ifs_ctx *c, Fmm **out) {
             }
         f->counts[0] = (int64_t)f->p2m.tasks.size();
     }
-    // M2M: child (level l+1) -> parent (level l)
+    // M2M: child (level l+1) -> parent (level l), one class per child octant;
+    // d = centre(child) - centre(parent) = (b - 1/2) * edge(child)
     f->m2m.resize(lmax + 1);
     for (int l = lmax - 1; l >= lmin; --l) {
         const Level &P = f->lv[l], &Ch = f->lv[l + 1];
-        std::vector<std::vector<int>> kids(P.boxes.size());
-        for (size_t q = 0; q < Ch.boxes.size(); ++q) kids[P.index.at(tree::parent(Ch.boxes[q]))].push_back((int)q);
-        for (size_t p = 0; p < P.boxes.size(); ++p)
-            for (int q : kids[p]) {
-                double dv[3];
-                dvec(&Ch.centre[3 * q], &P.centre[3 * p], dv);
-                f->m2m[l].tasks.push_back(
-                    make_task(dv, k, 1, Ch.L, P.L, Ch.off + q * Ch.per_box(), P.off + p * P.per_box()));
-            }
-        f->counts[1] += (int64_t)f->m2m[l].tasks.size();
+        std::vector<std::vector<std::pair<int64_t, int64_t>>> oct(8);
+        for (size_t q = 0; q < Ch.boxes.size(); ++q) {
+            int p = P.index.at(tree::parent(Ch.boxes[q]));
+            oct[Ch.boxes[q] & 7].push_back({Ch.off + (int64_t)q * Ch.per_box(), P.off + (int64_t)p * P.per_box()});
+        }
+        for (int b = 0; b < 8; ++b) {
+            double dv[3] = {(((b >> 2) & 1) - 0.5) * Ch.edge, (((b >> 1) & 1) - 0.5) * Ch.edge, ((b & 1) - 0.5) * Ch.edge};
+            std::sort(oct[b].begin(), oct[b].end(), [](auto &x, auto &y) { return x.second < y.second; });
+            add_class(f->m2m[l], dv, k, 1, Ch.L, P.L, oct[b]);
+            f->counts[1] += (int64_t)oct[b].size();
+        }
     }
-    // M2L: I4 of every box, sources ascending
+    // M2L: I4 of every box, one class per integer offset o = coords(src) - coords(tgt);
+    // d = o * edge.  Classes ordered by |o|^2, then o_z, so consecutive classes share tables.
     f->m2l.resize(lmax + 1);
     for (int l = lmin; l <= lmax; ++l) {
         const Level &V = f->lv[l];
+        std::map<std::array<int, 3>, std::vector<std::pair<int64_t, int64_t>>> cls;
         for (size_t q = 0; q < V.boxes.size(); ++q) {
+            int64_t vt[3];
+            box_coords(V.boxes[q], l, vt);
             std::vector<int64_t> i4 = tree::domain_i4(V.boxes[q], l, 1);
             for (int64_t src : i4) {
                 auto it = V.index.find(src);
                 if (it == V.index.end()) continue;
                 int p = it->second;
-                double dv[3];
-                dvec(&V.centre[3 * p], &V.centre[3 * q], dv);
-                f->m2l[l].tasks.push_back(
-                    make_task(dv, k, c->prm.z_selector, V.L, V.L, V.off + p * V.per_box(), V.off + q * V.per_box()));
+                int64_t vs[3];
+                box_coords(src, l, vs);
+                std::array<int, 3> o = {(int)(vs[0] - vt[0]), (int)(vs[1] - vt[1]), (int)(vs[2] - vt[2])};
+                cls[o].push_back({V.off + (int64_t)p * V.per_box(), V.off + (int64_t)q * V.per_box()});
             }
         }
-        f->counts[2] += (int64_t)f->m2l[l].tasks.size();
+        std::vector<std::array<int, 3>> keys;
+        for (auto &kv : cls) keys.push_back(kv.first);
+        std::sort(keys.begin(), keys.end(), [](const std::array<int, 3> &a, const std::array<int, 3> &b) {
+            int ra = a[0] * a[0] + a[1] * a[1] + a[2] * a[2], rb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            if (ra != rb) return ra < rb;
+            if (a[2] != b[2]) return a[2] < b[2];
+            return a < b;
+        });
+        for (auto &o : keys) {
+            auto &cols = cls[o];
+            std::sort(cols.begin(), cols.end(), [](auto &x, auto &y) { return x.second < y.second; });
+            double dv[3] = {o[0] * V.edge, o[1] * V.edge, o[2] * V.edge};
+            add_class(f->m2l[l], dv, k, c->prm.z_selector, V.L, V.L, cols);
+            f->counts[2] += (int64_t)cols.size();
+        }
     }
-    // L2L: parent (level l-1) -> child (level l)
+    // L2L: parent (level l-1) -> child (level l), one class per child octant;
+    // d = centre(parent) - centre(child) = -(b - 1/2) * edge(child)
     f->l2l.resize(lmax + 1);
     for (int l = lmin + 1; l <= lmax; ++l) {
         const Level &P = f->lv[l - 1], &Ch = f->lv[l];
+        std::vector<std::vector<std::pair<int64_t, int64_t>>> oct(8);
         for (size_t q = 0; q < Ch.boxes.size(); ++q) {
             int p = P.index.at(tree::parent(Ch.boxes[q]));
-            double dv[3];
-            dvec(&P.centre[3 * p], &Ch.centre[3 * q], dv);
-            f->l2l[l].tasks.push_back(
-                make_task(dv, k, 1, P.L, Ch.L, P.off + p * P.per_box(), Ch.off + q * Ch.per_box()));
+            oct[Ch.boxes[q] & 7].push_back({P.off + (int64_t)p * P.per_box(), Ch.off + (int64_t)q * Ch.per_box()});
         }
-        f->counts[3] += (int64_t)f->l2l[l].tasks.size();
+        for (int b = 0; b < 8; ++b) {
+            double dv[3] = {-(((b >> 2) & 1) - 0.5) * Ch.edge, -(((b >> 1) & 1) - 0.5) * Ch.edge,
+                            -((b & 1) - 0.5) * Ch.edge};
+            add_class(f->l2l[l], dv, k, 1, P.L, Ch.L, oct[b]);
+            f->counts[3] += (int64_t)oct[b].size();
+        }
     }
     // L2P: leaf box -> sphere (targets ascending)
     for (int i = 0; i < ns; ++i) {
@@ -222,11 +281,17 @@ int build(ifs_ctx *c, Fmm **out) {
         if (!r) batch_cost(B, c->flops_mv, c->bytes_mv);
         return r;
     };
+    auto cfin = [&](ClassBatch &B, const char *name) -> int {
+        B.name = name;
+        int r = class_batch_finalize(c, B);
+        if (!r) class_batch_cost(B, c->flops_mv, c->bytes_mv);
+        return r;
+    };
     if ((rc = fin(f->p2m, "p2m"))) goto fail;
     for (int l = lmin; l <= lmax; ++l) {
-        if (l < lmax && (rc = fin(f->m2m[l], "m2m"))) goto fail;
-        if ((rc = fin(f->m2l[l], "m2l"))) goto fail;
-        if (l > lmin && (rc = fin(f->l2l[l], "l2l"))) goto fail;
+        if (l < lmax && (rc = cfin(f->m2m[l], "m2m"))) goto fail;
+        if ((rc = cfin(f->m2l[l], "m2l"))) goto fail;
+        if (l > lmin && (rc = cfin(f->l2l[l], "l2l"))) goto fail;
     }
     if ((rc = fin(f->l2p, "l2p"))) goto fail;
     if ((rc = fin(f->p2p, "p2p"))) goto fail;
@@ -246,10 +311,10 @@ int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc) {
     int rc;
     if ((rc = batch_run(c, f->p2m, xh, C))) return rc;
     for (int l = f->l_max - 1; l >= f->l_min; --l)
-        if ((rc = batch_run(c, f->m2m[l], C, C))) return rc;
+        if ((rc = class_batch_run(c, f->m2m[l], C, C))) return rc;
     for (int l = f->l_min; l <= f->l_max; ++l) {
-        if ((rc = batch_run(c, f->m2l[l], C, D))) return rc;
-        if (l > f->l_min && (rc = batch_run(c, f->l2l[l], D, D))) return rc;
+        if ((rc = class_batch_run(c, f->m2l[l], C, D))) return rc;
+        if (l > f->l_min && (rc = class_batch_run(c, f->l2l[l], D, D))) return rc;
     }
     if ((rc = batch_run(c, f->l2p, D, acc))) return rc;
     if ((rc = batch_run(c, f->p2p, xh, acc))) return rc;

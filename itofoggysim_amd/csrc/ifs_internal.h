@@ -34,6 +34,18 @@ struct Task {
     int32_t kind;    // 1: regular (j_n), 3: outgoing -> regular (h1_n)
 };
 
+// A translation class: columns [col_begin, col_begin + ncols) of the class
+// source/target offset arrays share one translation vector and truncations.
+struct ClassDesc {
+    int64_t off_d;    // Wigner-d table (doubles) in the d workspace
+    int64_t off_ax;   // coaxial table (complex) in the coaxial workspace
+    double phi;       // azimuth of the translation vector
+    int32_t Ls, Lt, K;
+    int32_t ncols;
+    int32_t col_begin;
+    int32_t pad;
+};
+
 // A run of consecutive tasks that share one target expansion.
 struct Segment {
     int64_t dst;     // complex offset of the target expansion
@@ -98,6 +110,10 @@ void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const d
                         double2 *wz, hipStream_t s);
 void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblocks, const double *wd,
                      const double2 *wz, double2 *dst_pool, hipStream_t s);
+
+void launch_class(const ClassDesc &h, const ClassDesc *d_cls, const int64_t *src_off, const int64_t *dst_off,
+                  const double2 *src_pool, double2 *dst_pool, const double *wd, const double2 *wax, double *wy,
+                  double *wz, int stage, hipStream_t s);
 
 void launch_mie(int ns, const double *xsize, const double2 *mrel, int N, double2 *an, double2 *bn, hipStream_t s);
 void launch_pack_in(int ns, int N, const int32_t *Leff, const int64_t *off, const double *snm, const double2 *x,
