@@ -1,0 +1,155 @@
+"""Benchmark: MLFMM matvecs/s of the 2000-sphere cluster on one MI355X.
+
+One step = one matvec b = (I + T A) x of the 2000-water-sphere cluster (lognormal
+radii, median 1 um, sigma_g 1.5, clipped to [0.2, 3] um, volume fraction 1e-3,
+lambda 550 nm, seed 2000) through the MLFMM (leaf <= 32 spheres, box truncation
+L = kD + 1.8 d0^(2/3) (kD)^(1/3), d0 = 6), FP64, inputs resident in HBM.
+
+Prints one JSON line (rank 0).  The CPU baseline times the oracle (the
+reference's algorithm: full Gaunt/3j translation matrices per sphere pair,
+regenerated every matvec) on a bounded sample of sphere pairs of the same
+cluster and extrapolates to one direct matvec over all ordered pairs.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "MLFMM matvecs/s & solves/s at rel. residual 1e-6; p50/p99 solve latency"
+
+
+def cpu_baseline(cluster, npairs, threads):
+    from oracle import oracle as O
+    ns = len(cluster["radii"])
+    N = int(cluster["lmax"].max())
+    rng = np.random.default_rng(12345)
+    tgt = rng.integers(0, ns, npairs).astype(np.int32)
+    src = ((tgt + 1 + rng.integers(0, ns - 1, npairs)) % ns).astype(np.int32)
+    an = np.full((ns, N), 0.5 + 0.1j)
+    bn = an.copy()
+    x = np.ones(ns * 2 * N * (N + 2), dtype=np.complex128)
+    t0 = time.perf_counter()
+    O.matvec_pairs(cluster["centres"], cluster["lmax"], N, an, bn, cluster["k"], x, tgt, src, nthreads=threads)
+    dt = time.perf_counter() - t0
+    pairs_per_s = npairs / dt
+    return {"value": pairs_per_s / (ns * (ns - 1)), "unit": "matvecs/s", "cores": threads, "kind": "port",
+            "sample": f"{npairs} random ordered sphere pairs of the same 2000-sphere cluster through the oracle "
+                      f"(full Xu A/B by the Gaunt/3j formula per pair, dense apply) in {dt:.1f} s on {threads} "
+                      f"threads, extrapolated to one direct matvec over all {ns * (ns - 1)} ordered pairs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-pairs", type=int, default=160)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solve", action="store_true", help="also run GMRES(50) to 1e-6")
+    ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from itofoggysim_amd import MultiSphereSolver, config_cluster
+
+    c = config_cluster(2, n=args.n if args.n != 2000 else None)
+    t0 = time.perf_counter()
+    s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"], use_fmm=True,
+                          leaf_size=32, fmm_d0=6.0)
+    build_s = time.perf_counter() - t0
+    counts, levels = s.fmm_stats()
+    flops_mv, bytes_mv = s.matvec_cost()
+    print(f"# built: {s.n_spheres} spheres, N={s.n_global}, vector {s.n}, levels L={levels}, {counts}, "
+          f"{build_s:.2f} s", file=sys.stderr, flush=True)
+
+    st = torch.cuda.Stream()
+    gen = torch.Generator(device="cpu").manual_seed(2000)
+    x = torch.randn(s.n, dtype=torch.complex128, generator=gen).to("cuda")
+    b = torch.empty_like(x)
+    with torch.cuda.stream(st):
+        s.set_stream(st.cuda_stream)
+        for i in range(args.warmup):
+            s.calculate_vector_b_device(x.data_ptr(), b.data_ptr())
+            st.synchronize()
+            print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(st)
+        for i in range(args.steps):
+            s.calculate_vector_b_device(x.data_ptr(), b.data_ptr())
+        e1.record(st)
+        st.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        ms = e0.elapsed_time(e1) / args.steps
+        print(f"# timed {args.steps} steps: {ms:.1f} ms/step (wall {wall:.2f} s)", file=sys.stderr, flush=True)
+        # per-kernel profile of one extra matvec (HIP events on the launch stream)
+        s.profile(True)
+        s.calculate_vector_b_device(x.data_ptr(), b.data_ptr())
+        prof = s.profile_read()
+        s.profile(False)
+    finite = bool(torch.isfinite(b).all().item())
+
+    kern = {k: v for k, v in prof.items() if v[2] > 0}
+    dom = max(kern, key=lambda k: kern[k][0])
+    tot_ms, launches, fl = kern[dom]
+    achieved = fl / (tot_ms * 1e-3) / 1e12
+    # FP64 dense products; MI355X's FP64 matrix and vector peaks are both 78.6 TFLOP/s
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                "traffic": None, "launches": launches, "avg_launch_ms": round(tot_ms / launches, 4),
+                "algorithmic_flops_per_launch": fl / launches,
+                "matvec_algorithmic_tflops": round(flops_mv / (ms * 1e-3) / 1e12, 3)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(1000.0 / ms, 6),
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cluster generator, seed 2000; random seeded x)",
+        "config": {
+            "workload": "MLFMM matvec of the 2000-sphere water cluster (lognormal r, median 1 um, sigma_g 1.5, "
+                        "[0.2,3] um, volume fraction 1e-3, lambda 550 nm), leaf <= 32, box L = kD + 1.8 d0^(2/3)"
+                        " (kD)^(1/3) with d0 = 6, FP64",
+            "n_spheres": s.n_spheres, "unknowns_flat": s.n, "box_L_per_level": levels, "translations": counts,
+            "matvec_algorithmic_flop": flops_mv, "build_s": round(build_s, 3)},
+        "roofline": roofline,
+        "kernels_ms": {k: round(v[0], 2) for k, v in prof.items()},
+        "finite": finite,
+    }
+    if args.solve:
+        rhs = torch.from_numpy(s.get_vector_b()).to("cuda")
+        xs = torch.zeros_like(rhs)
+        with torch.cuda.stream(st):
+            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=400, tol=1e-6)
+        out["solve"] = info
+        out["solves_per_s"] = 1.0 / info["seconds"] if info["converged"] else None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(c, args.cpu_pairs, threads)
+    s.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

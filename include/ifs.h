@@ -138,8 +138,10 @@ int ifs_fmm_stats(ifs_ctx *ctx, int64_t counts[6], int32_t levels_L[32]);
 /* Per-kernel device time of the last profiled calls: enable = 1 starts
  * recording HIP events around every launch on the context stream. */
 int ifs_profile_enable(ifs_ctx *ctx, int32_t enable);
-/* Returns the number of kernel kinds written (<= cap); names are static strings. */
-int ifs_profile_read(ifs_ctx *ctx, const char **names, double *total_ms, int64_t *launches, int32_t cap);
+/* Returns the number of kernel kinds written (<= cap); names are static strings;
+ * flops is the algorithmic flop count of the recorded launches (0 where not defined). */
+int ifs_profile_read(ifs_ctx *ctx, const char **names, double *total_ms, int64_t *launches, double *flops,
+                     int32_t cap);
 /* Algorithmic flop count of one matvec (rotation form (128/3) L^3 per translation,
  * plus T-apply and identity) and compulsory bytes, from the actual plan. */
 int ifs_matvec_cost(ifs_ctx *ctx, double *flops, double *bytes);

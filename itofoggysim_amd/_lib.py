@@ -73,7 +73,7 @@ SIGNATURES = {
     "ifs_tree_level_max": (_i32, [_vp, _i64, _i64]),
     "ifs_fmm_stats": (_i32, [_vp, _vp, _vp]),
     "ifs_profile_enable": (_i32, [_vp, _i32]),
-    "ifs_profile_read": (_i32, [_vp, _vp, _vp, _vp, _i32]),
+    "ifs_profile_read": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32]),
     "ifs_matvec_cost": (_i32, [_vp, _vp, _vp]),
 }
 

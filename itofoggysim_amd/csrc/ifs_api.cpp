@@ -179,7 +179,7 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
     return IFS_OK;
 }
 
-Scope::Scope(ifs_ctx *c_, const char *n) : c(c_), name(n) {
+Scope::Scope(ifs_ctx *c_, const char *n, double algorithmic_flops) : c(c_), name(n), flops(algorithmic_flops) {
     if (c->prof.on) {
         a = c->prof.get();
         (void)hipEventRecord(a, c->stream);
@@ -189,7 +189,7 @@ Scope::~Scope() {
     if (c->prof.on) {
         b = c->prof.get();
         (void)hipEventRecord(b, c->stream);
-        c->prof.recs.push_back({name, a, b});
+        c->prof.recs.push_back({name, a, b, flops});
     }
 }
 
@@ -288,6 +288,15 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
     double *wz = c->ws_z.as<double>();
     for (size_t i = 0; i < B.cls.size(); ++i) {
         const Task &T = B.gen[i];
+        double ffwd = 0.0, fback = 0.0, fax = 0.0;
+        if (c->prof.on) {
+            for (int n = 1; n <= T.Ls; ++n) ffwd += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+            for (int n = 1; n <= T.Lt; ++n) fback += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+            for (int k = -T.K; k <= T.K; ++k) fax += 32.0 * (double)ax_block(k, T.Lt, T.Ls);
+            ffwd *= B.cls[i].ncols;
+            fback *= B.cls[i].ncols;
+            fax *= B.cls[i].ncols;
+        }
         if (B.regen_d[i]) {
             Scope s(c, "class_gen_wigner");
             int L = std::max(T.Ls, T.Lt);
@@ -299,17 +308,17 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
             launch_gen_axial(dg + i, 1, T.K, T.Lt + T.Ls + 2, wsc, wax, c->stream);
         }
         {
-            Scope s(c, "class_rot_fwd");
+            Scope s(c, "class_rot_fwd", ffwd);
             launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
                          wy, wz, 0, c->stream);
         }
         {
-            Scope s(c, "class_axial");
+            Scope s(c, "class_axial", fax);
             launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
                          wy, wz, 1, c->stream);
         }
         {
-            Scope s(c, "class_rot_back");
+            Scope s(c, "class_rot_back", fback);
             launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
                          wy, wz, 2, c->stream);
         }
@@ -579,21 +588,17 @@ int ifs_profile_enable(ifs_ctx *c, int32_t enable) {
     return IFS_OK;
 }
 
-int ifs_profile_read(ifs_ctx *c, const char **names, double *total_ms, int64_t *launches, int32_t cap) {
-    if (!c) return IFS_E_ARG;
+int ifs_profile_read(ifs_ctx *c, const char **names, double *total_ms, int64_t *launches, double *flops,
+                     int32_t cap) {
+    if (!c || !names || !total_ms || !launches || !flops) return IFS_E_ARG;
     c->prof.flush();
     int i = 0;
-    static std::vector<std::string> keep;  // stable storage for returned names
     for (auto &kv : c->prof.totals) {
         if (i >= cap) break;
-        bool found = false;
-        for (auto &s : keep)
-            if (s == kv.first) found = true;
-        if (!found) keep.push_back(kv.first);
-        for (auto &s : keep)
-            if (s == kv.first) names[i] = s.c_str();
-        total_ms[i] = kv.second.first;
-        launches[i] = kv.second.second;
+        names[i] = kv.first;
+        total_ms[i] = kv.second.ms;
+        launches[i] = kv.second.launches;
+        flops[i] = kv.second.flops;
         ++i;
     }
     return i;

@@ -62,12 +62,18 @@ struct Batch {
 struct Profiler {
     bool on = false;
     struct Rec {
-        const char *name;
+        const char *name;  // static string
         hipEvent_t a, b;
+        double flops;
+    };
+    struct Tot {
+        double ms = 0.0;
+        int64_t launches = 0;
+        double flops = 0.0;
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
-    std::map<std::string, std::pair<double, int64_t>> totals;
+    std::map<const char *, Tot> totals;  // keyed by the static name
     hipEvent_t get() {
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
@@ -83,9 +89,10 @@ struct Profiler {
             (void)hipEventSynchronize(r.b);
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, r.a, r.b);
-            auto &t = totals[r.name];
-            t.first += ms;
-            t.second += 1;
+            Tot &t = totals[r.name];
+            t.ms += ms;
+            t.launches += 1;
+            t.flops += r.flops;
             pool.push_back(r.a);
             pool.push_back(r.b);
         }
@@ -163,7 +170,8 @@ struct Scope {
     ifs_ctx *c;
     const char *name;
     hipEvent_t a{}, b{};
-    Scope(ifs_ctx *c_, const char *n);
+    double flops;
+    Scope(ifs_ctx *c_, const char *n, double algorithmic_flops = 0.0);
     ~Scope();
 };
 }  // namespace ifs

@@ -22,11 +22,12 @@ namespace ifs {
 
 namespace {
 
-constexpr int BM = 64;   // rows per tile
-constexpr int BN = 64;   // real columns per tile (16 translations)
-constexpr int BK = 16;   // inner step
-constexpr int TM = 4;    // rows per thread
-constexpr int TN = 4;    // columns per thread (one translation: a.re a.im b.re b.im)
+// Real products (rotations): 128 x 128 tiles, 256 threads, 8 x 8 outputs per thread.
+// Thread (ty, tx) owns rows ty + 16 i (i < 8) and the 2 translations tx + 16 j
+// (j < 2), i.e. the 8 real columns 4 (tx + 16 j) + p.
+constexpr int BM = 128;
+constexpr int BN = 128;  // 32 translations
+constexpr int BK = 8;
 
 __device__ __forceinline__ int fidx(int n, int m) { return n * n - 1 + m + n; }
 
@@ -42,86 +43,95 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restri
     if (nu > C.Ls) return;
     const int K = C.K;
     const int kk = nu < K ? nu : K;
-    const int M = 2 * kk + 1;        // rows k = -kk..kk
-    const int KI = 2 * nu + 1;       // inner m = -nu..nu
+    const int M = 2 * kk + 1;   // rows k = -kk..kk
+    const int KI = 2 * nu + 1;  // inner m = -nu..nu
     const int NC = 4 * C.ncols;
     const int row0 = blockIdx.y * BM;
     const int col0 = blockIdx.x * BN;
     if (row0 >= M) return;
-    __shared__ double As[BK][BM + 1];
-    __shared__ double Bs[BK][BN + 1];
+    __shared__ double As[BK][BM];
+    __shared__ double Bs[BK][BN];
     const double *A = wd + C.off_d + wig_off(nu, K);  // row-major (M x KI)
     const int tid = threadIdx.x;
-    const int tr = tid / (BN / TN);  // 0..15
-    const int tc = tid % (BN / TN);  // 0..15
-    double acc[TM][TN];
+    const int ty = tid >> 4, tx = tid & 15;
+    double acc[8][8];
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
-    const int Lsrc = C.Ls;
-    const int64_t nsrc = (int64_t)Lsrc * (Lsrc + 2);
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
+    const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
+    // gather assignment: thread -> (inner row q, translation t) for the B tile
+    const int gq_l = tid >> 5, gt = tid & 31;
+    const int gj = (col0 >> 2) + gt;
+    const double2 *xs = (gj < C.ncols) ? src_pool + src_off[C.col_begin + gj] : nullptr;
     for (int k0 = 0; k0 < KI; k0 += BK) {
-        // A tile: BM x BK (1024 values, 4 per thread)
-        for (int e = tid; e < BM * BK; e += 256) {
-            int r = e / BK, q = e % BK;
-            int gr = row0 + r, gq = k0 + q;
-            As[q][r] = (gr < M && gq < KI) ? A[(int64_t)gr * KI + gq] : 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;       // 0..1023 over 128 rows x 8 inner
+            int r = idx >> 3, q = idx & 7;
+            int gr = row0 + r, gqq = k0 + q;
+            As[q][r] = (gr < M && gqq < KI) ? A[(int64_t)gr * KI + gqq] : 0.0;
         }
-        // B tile: BK x BN, fused gather with phase: one thread per (m, translation)
-        for (int e = tid; e < BK * (BN / 4); e += 256) {
-            int q = e / (BN / 4), t = e % (BN / 4);
-            int gq = k0 + q;
-            int j = (col0 / 4) + t;
+        {
+            int gqq = k0 + gq_l;
             double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-            if (gq < KI && j < C.ncols) {
-                int m = gq - nu;
-                const double2 *x = src_pool + src_off[C.col_begin + j] + fidx(nu, m);
+            if (gqq < KI && xs) {
+                int m = gqq - nu;
+                const double2 *x = xs + fidx(nu, m);
                 double2 xa = x[0], xb = x[nsrc];
-                double s, c;
-                sincos((double)m * C.phi, &s, &c);
-                v0 = c * xa.x - s * xa.y;
-                v1 = c * xa.y + s * xa.x;
-                v2 = c * xb.x - s * xb.y;
-                v3 = c * xb.y + s * xb.x;
+                double sn, cs;
+                sincos((double)m * C.phi, &sn, &cs);
+                v0 = cs * xa.x - sn * xa.y;
+                v1 = cs * xa.y + sn * xa.x;
+                v2 = cs * xb.x - sn * xb.y;
+                v3 = cs * xb.y + sn * xb.x;
             }
-            Bs[q][4 * t + 0] = v0;
-            Bs[q][4 * t + 1] = v1;
-            Bs[q][4 * t + 2] = v2;
-            Bs[q][4 * t + 3] = v3;
+            Bs[gq_l][4 * gt + 0] = v0;
+            Bs[gq_l][4 * gt + 1] = v1;
+            Bs[gq_l][4 * gt + 2] = v2;
+            Bs[gq_l][4 * gt + 3] = v3;
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < BK; ++q) {
-            double a[TM], b[TN];
+            double a[8], b[8];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = As[q][tr * TM + i];
+            for (int i = 0; i < 8; ++i) a[i] = As[q][ty + 16 * i];
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bs[q][tc * TN + j];
+            for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int i = 0; i < TM; ++i)
+                for (int p = 0; p < 4; ++p) b[4 * j + p] = Bs[q][4 * (tx + 16 * j) + p];
 #pragma unroll
-                for (int j = 0; j < TN; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
     double *Y = wy + rows_off(nu, K) * (int64_t)NC;
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-        int gr = row0 + tr * TM + i;
+    for (int i = 0; i < 8; ++i) {
+        int gr = row0 + ty + 16 * i;
         if (gr >= M) continue;
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            int gc = col0 + tc * TN + j;
-            if (gc < NC) Y[(int64_t)gr * NC + gc] = acc[i][j];
+        for (int j = 0; j < 2; ++j) {
+            int t = (col0 >> 2) + tx + 16 * j;
+            if (t >= C.ncols) continue;
+            double *y = Y + (int64_t)gr * NC + 4 * t;
+            y[0] = acc[i][4 * j + 0];
+            y[1] = acc[i][4 * j + 1];
+            y[2] = acc[i][4 * j + 2];
+            y[3] = acc[i][4 * j + 3];
         }
     }
 }
 
 // Coaxial shift of one class: grid (translation tiles, row tiles, 2K+1 orders).
 // Z_k^a(n) = sum_nu A_k(n,nu) Y_k^a(nu) + B_k(n,nu) Y_k^b(nu);  Z_k^b = B Y^a + A Y^b
-constexpr int AM = 32;   // rows n per tile
-constexpr int AN = 32;   // translations per tile
+// 64 rows x 64 translations per tile; thread (ty, tx) owns rows ty + 16 i and
+// translations tx + 16 j (i, j < 4).
+constexpr int AM = 64;
+constexpr int AN = 64;
 constexpr int AK = 8;
 
 __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict__ cls,
@@ -143,29 +153,32 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
     const double2 *Bb = Ab + ax_block(k, Lt, Ls);
     __shared__ double2 As[AK][AM];
     __shared__ double2 Bs2[AK][AM];
-    __shared__ double2 Ya[AK][AN + 1];
-    __shared__ double2 Yb[AK][AN + 1];
+    __shared__ double2 Ya[AK][AN];
+    __shared__ double2 Yb[AK][AN];
     const int tid = threadIdx.x;
-    const int tr = tid / 16;  // 0..15 -> 2 rows each
-    const int tc = tid % 16;  // 0..15 -> 2 translations each
-    double2 za[2][2], zb[2][2];
+    const int ty = tid >> 4, tx = tid & 15;
+    double2 za[4][4], zb[4][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 4; ++j) {
             za[i][j] = make_double2(0.0, 0.0);
             zb[i][j] = make_double2(0.0, 0.0);
         }
     for (int k0 = 0; k0 < KI; k0 += AK) {
-        for (int e = tid; e < AM * AK; e += 256) {
-            int r = e / AK, c = e % AK;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            int idx = tid + 256 * e;  // 0..511 over 64 rows x 8 inner
+            int r = idx >> 3, c = idx & 7;
             int gr = row0 + r, gc = k0 + c;
             bool ok = gr < M && gc < KI;
             As[c][r] = ok ? Ab[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
             Bs2[c][r] = ok ? Bb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
         }
-        for (int e = tid; e < AK * AN; e += 256) {
-            int c = e / AN, t = e % AN;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            int idx = tid + 256 * e;  // 0..511 over 8 inner x 64 translations
+            int c = idx >> 6, t = idx & 63;
             int nu = q + k0 + c;
             int j = col0 + t;
             double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
@@ -181,22 +194,21 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < AK; ++c) {
-            double2 a[2], b[2], ya[2], yb[2];
+            double2 a[4], b[4], ya[4], yb[4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = As[c][tr * 2 + i];
-                b[i] = Bs2[c][tr * 2 + i];
+            for (int i = 0; i < 4; ++i) {
+                a[i] = As[c][ty + 16 * i];
+                b[i] = Bs2[c][ty + 16 * i];
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                ya[j] = Ya[c][tc * 2 + j];
-                yb[j] = Yb[c][tc * 2 + j];
+            for (int j = 0; j < 4; ++j) {
+                ya[j] = Ya[c][tx + 16 * j];
+                yb[j] = Yb[c][tx + 16 * j];
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    // za += a*ya + b*yb ; zb += b*ya + a*yb
+                for (int j = 0; j < 4; ++j) {
                     za[i][j].x = fma(a[i].x, ya[j].x, za[i][j].x);
                     za[i][j].x = fma(-a[i].y, ya[j].y, za[i][j].x);
                     za[i][j].y = fma(a[i].x, ya[j].y, za[i][j].y);
@@ -218,14 +230,14 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        int n = q + row0 + tr * 2 + i;
+    for (int i = 0; i < 4; ++i) {
+        int n = q + row0 + ty + 16 * i;
         if (n > Lt) continue;
         int kk = n < K ? n : K;
         double *row = wz + (rows_off(n, K) + (k + kk)) * (int64_t)NC;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            int t = col0 + tc * 2 + j;
+        for (int j = 0; j < 4; ++j) {
+            int t = col0 + tx + 16 * j;
             if (t >= C.ncols) continue;
             row[4 * t + 0] = za[i][j].x;
             row[4 * t + 1] = za[i][j].y;
@@ -252,62 +264,71 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const ClassDesc *__restr
     const int row0 = blockIdx.y * BM;
     const int col0 = blockIdx.x * BN;
     if (row0 >= M) return;
-    __shared__ double As[BK][BM + 1];
-    __shared__ double Bs[BK][BN + 1];
+    __shared__ double As[BK][BM];
+    __shared__ double Bs[BK][BN];
     const double *D = wd + C.off_d + wig_off(n, K);  // (KI x M) row-major: D[k][m]
     const double *Z = wz + rows_off(n, K) * (int64_t)NC;
     const int tid = threadIdx.x;
-    const int tr = tid / (BN / TN);
-    const int tc = tid % (BN / TN);
-    double acc[TM][TN];
+    const int ty = tid >> 4, tx = tid & 15;
+    double acc[8][8];
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
     for (int k0 = 0; k0 < KI; k0 += BK) {
-        for (int e = tid; e < BM * BK; e += 256) {
-            int q = e / BM, r = e % BM;  // coalesced along m
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;  // 8 inner x 128 rows, coalesced along m
+            int q = idx >> 7, r = idx & 127;
             int gr = row0 + r, gq = k0 + q;
             As[q][r] = (gr < M && gq < KI) ? D[(int64_t)gq * M + gr] : 0.0;
         }
-        for (int e = tid; e < BK * BN; e += 256) {
-            int q = e / BN, cc = e % BN;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;  // 8 inner x 128 columns
+            int q = idx >> 7, cc = idx & 127;
             int gq = k0 + q, gc = col0 + cc;
             Bs[q][cc] = (gq < KI && gc < NC) ? Z[(int64_t)gq * NC + gc] : 0.0;
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < BK; ++q) {
-            double a[TM], b[TN];
+            double a[8], b[8];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = As[q][tr * TM + i];
+            for (int i = 0; i < 8; ++i) a[i] = As[q][ty + 16 * i];
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bs[q][tc * TN + j];
+            for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int i = 0; i < TM; ++i)
+                for (int p = 0; p < 4; ++p) b[4 * j + p] = Bs[q][4 * (tx + 16 * j) + p];
 #pragma unroll
-                for (int j = 0; j < TN; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
-    // epilogue: thread owns rows m and exactly one translation (TN = 4 reals)
-    const int j = (col0 + tc * TN) / 4;
-    if (j >= C.ncols) return;
-    double2 *dst = dst_pool + dst_off[C.col_begin + j];
     const int64_t ndst = (int64_t)C.Lt * (C.Lt + 2);
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-        int gr = row0 + tr * TM + i;
-        if (gr >= M) continue;
-        int m = gr - n;
-        double s, c;
-        sincos(-(double)m * C.phi, &s, &c);
-        double2 wa = make_double2(c * acc[i][0] - s * acc[i][1], c * acc[i][1] + s * acc[i][0]);
-        double2 wb = make_double2(c * acc[i][2] - s * acc[i][3], c * acc[i][3] + s * acc[i][2]);
-        int64_t o = fidx(n, m);
-        double2 da = dst[o], db = dst[o + ndst];
-        dst[o] = make_double2(da.x + wa.x, da.y + wa.y);
-        dst[o + ndst] = make_double2(db.x + wb.x, db.y + wb.y);
+    for (int j = 0; j < 2; ++j) {
+        int t = (col0 >> 2) + tx + 16 * j;
+        if (t >= C.ncols) continue;
+        double2 *dst = dst_pool + dst_off[C.col_begin + t];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            int gr = row0 + ty + 16 * i;
+            if (gr >= M) continue;
+            int m = gr - n;
+            double sn, cs;
+            sincos(-(double)m * C.phi, &sn, &cs);
+            double2 wa = make_double2(cs * acc[i][4 * j] - sn * acc[i][4 * j + 1],
+                                      cs * acc[i][4 * j + 1] + sn * acc[i][4 * j]);
+            double2 wb = make_double2(cs * acc[i][4 * j + 2] - sn * acc[i][4 * j + 3],
+                                      cs * acc[i][4 * j + 3] + sn * acc[i][4 * j + 2]);
+            int64_t o = fidx(n, m);
+            double2 da = dst[o], db = dst[o + ndst];
+            dst[o] = make_double2(da.x + wa.x, da.y + wa.y);
+            dst[o + ndst] = make_double2(db.x + wb.x, db.y + wb.y);
+        }
     }
 }
 

@@ -108,14 +108,16 @@ class MultiSphereSolver:
         check(lib().ifs_profile_enable(self._h, int(bool(enable))), "ifs_profile_enable")
 
     def profile_read(self, cap=64):
+        """{kernel: (total ms, launches, algorithmic flops)} since profile(True)."""
         names = (ctypes.c_char_p * cap)()
         ms = (ctypes.c_double * cap)()
         cnt = (ctypes.c_int64 * cap)()
+        fl = (ctypes.c_double * cap)()
         n = lib().ifs_profile_read(self._h, ctypes.cast(names, ctypes.c_void_p), ctypes.cast(ms, ctypes.c_void_p),
-                                   ctypes.cast(cnt, ctypes.c_void_p), cap)
+                                   ctypes.cast(cnt, ctypes.c_void_p), ctypes.cast(fl, ctypes.c_void_p), cap)
         if n < 0:
             check(n, "ifs_profile_read")
-        return {names[i].decode(): (ms[i], cnt[i]) for i in range(n)}
+        return {names[i].decode(): (ms[i], cnt[i], fl[i]) for i in range(n)}
 
     def matvec_cost(self):
         f = ctypes.c_double()
