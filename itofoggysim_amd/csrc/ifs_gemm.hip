@@ -1,4 +1,4 @@
-// Class-batched translations as tiled FP64 GEMMs (gfx950).
+// Class-batched translations as tiled FP64 matrix-core (MFMA) products (gfx950).
 //
 // A translation class is a set of translations that share one translation
 // vector (the same octree offset at one level, e.g. one M2L offset or one child
@@ -22,18 +22,39 @@ namespace ifs {
 
 namespace {
 
-// Real products (rotations): 128 x 128 tiles, 256 threads, 8 x 8 outputs per thread.
-// Thread (ty, tx) owns rows ty + 16 i (i < 8) and the 2 translations tx + 16 j
-// (j < 2), i.e. the 8 real columns 4 (tx + 16 j) + p.
-constexpr int BM = 128;
-constexpr int BN = 128;  // 32 translations
-constexpr int BK = 8;
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
+// FP64 matrix-core tile: D(16x16) += A(16x4) B(4x16) with lane l holding
+// A[l%16][l/16], B[l/16][l%16] and D[4v+l/16][l%16], v = 0..3.
+__device__ __forceinline__ dv4 mfma(double a, double b, dv4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 __device__ __forceinline__ int fidx(int n, int m) { return n * n - 1 + m + n; }
 
+// Rotation products: block tile 64 rows x 128 real columns (32 translations),
+// 4 waves of 32 x 64, inner step 16 (four 16x16x4 steps).
+constexpr int RM = 64;
+constexpr int RN = 128;
+constexpr int RK = 16;
+constexpr int RPAD = 2;
+
 }  // namespace
 
+// Phase table e^{i m phi}, m = -L..L, into LDS (ph[m + L]).
+__device__ __forceinline__ void phase_table(double2 *ph, int L, double phi, double sign) {
+    for (int m = threadIdx.x; m <= 2 * L; m += blockDim.x) {
+        double sn, cs;
+        sincos(sign * (double)(m - L) * phi, &sn, &cs);
+        ph[m] = make_double2(cs, sn);
+    }
+}
+
+constexpr int PH_MAX = 2 * 1024 + 1;  // phase table entries (L <= 1024)
+
 // Forward rotation of one class: grid (col tiles, row tiles, degree-1).
+//   Y^nu[k][4j+p] = sum_m d^nu_{km} (e^{i m phi} x_j(nu,m))_p
+// The next inner step is prefetched into registers while the matrix cores work.
 __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restrict__ cls,
                                                        const int64_t *__restrict__ src_off,
                                                        const double2 *__restrict__ src_pool,
@@ -46,90 +67,115 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restri
     const int M = 2 * kk + 1;   // rows k = -kk..kk
     const int KI = 2 * nu + 1;  // inner m = -nu..nu
     const int NC = 4 * C.ncols;
-    const int row0 = blockIdx.y * BM;
-    const int col0 = blockIdx.x * BN;
+    const int row0 = blockIdx.y * RM;
+    const int col0 = blockIdx.x * RN;
     if (row0 >= M) return;
-    __shared__ double As[BK][BM];
-    __shared__ double Bs[BK][BN];
+    __shared__ double As[RK][RM + RPAD];
+    __shared__ double Bs[RK][RN + RPAD];
+    __shared__ double2 ph[PH_MAX];
+    phase_table(ph, nu, C.phi, 1.0);
     const double *A = wd + C.off_d + wig_off(nu, K);  // row-major (M x KI)
     const int tid = threadIdx.x;
-    const int ty = tid >> 4, tx = tid & 15;
-    double acc[8][8];
+    const int lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+    dv4 acc[2][4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
     const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
-    // gather assignment: thread -> (inner row q, translation t) for the B tile
-    const int gq_l = tid >> 5, gt = tid & 31;
-    const int gj = (col0 >> 2) + gt;
-    const double2 *xs = (gj < C.ncols) ? src_pool + src_off[C.col_begin + gj] : nullptr;
-    for (int k0 = 0; k0 < KI; k0 += BK) {
+    // gather: thread -> (translation t, inner q): idx = tid + 256 e, t = idx / 16, q = idx % 16
+    const double2 *xs[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        int t = (tid + 256 * e) >> 4;
+        int j = (col0 >> 2) + t;
+        xs[e] = (j < C.ncols) ? src_pool + src_off[C.col_begin + j] + fidx(nu, 0) : nullptr;
+    }
+    __syncthreads();  // phase table ready
+    double ra[4];
+    double2 rxa[2], rxb[2];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;       // 0..1023 over 128 rows x 8 inner
-            int r = idx >> 3, q = idx & 7;
-            int gr = row0 + r, gqq = k0 + q;
-            As[q][r] = (gr < M && gqq < KI) ? A[(int64_t)gr * KI + gqq] : 0.0;
+            int idx = tid + 256 * e;  // 64 rows x 16 inner
+            int r = idx >> 4, q = idx & 15;
+            int gr = row0 + r, gq = k0 + q;
+            ra[e] = (gr < M && gq < KI) ? A[(int64_t)gr * KI + gq] : 0.0;
         }
-        {
-            int gqq = k0 + gq_l;
-            double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-            if (gqq < KI && xs) {
-                int m = gqq - nu;
-                const double2 *x = xs + fidx(nu, m);
-                double2 xa = x[0], xb = x[nsrc];
-                double sn, cs;
-                sincos((double)m * C.phi, &sn, &cs);
-                v0 = cs * xa.x - sn * xa.y;
-                v1 = cs * xa.y + sn * xa.x;
-                v2 = cs * xb.x - sn * xb.y;
-                v3 = cs * xb.y + sn * xb.x;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            int gq = k0 + ((tid + 256 * e) & 15);
+            if (gq < KI && xs[e]) {
+                rxa[e] = xs[e][gq - nu];
+                rxb[e] = xs[e][gq - nu + nsrc];
+            } else {
+                rxa[e] = make_double2(0.0, 0.0);
+                rxb[e] = make_double2(0.0, 0.0);
             }
-            Bs[gq_l][4 * gt + 0] = v0;
-            Bs[gq_l][4 * gt + 1] = v1;
-            Bs[gq_l][4 * gt + 2] = v2;
-            Bs[gq_l][4 * gt + 3] = v3;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < KI; k0 += RK) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;
+            As[idx & 15][idx >> 4] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            int idx = tid + 256 * e;
+            int t = idx >> 4, q = idx & 15;
+            int gq = k0 + q;
+            double2 p = (gq < KI) ? ph[gq] : make_double2(1.0, 0.0);  // m + nu = gq
+            double2 xa = rxa[e], xb = rxb[e];
+            Bs[q][4 * t + 0] = p.x * xa.x - p.y * xa.y;
+            Bs[q][4 * t + 1] = p.x * xa.y + p.y * xa.x;
+            Bs[q][4 * t + 2] = p.x * xb.x - p.y * xb.y;
+            Bs[q][4 * t + 3] = p.x * xb.y + p.y * xb.x;
         }
         __syncthreads();
+        if (k0 + RK < KI) load(k0 + RK);
 #pragma unroll
-        for (int q = 0; q < BK; ++q) {
-            double a[8], b[8];
+        for (int kc = 0; kc < RK; kc += 4) {
+            double a[2], b[4];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a[i] = As[q][ty + 16 * i];
+            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][wr + 16 * i + li];
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kc + lk][wc + 16 * j + li];
 #pragma unroll
-                for (int p = 0; p < 4; ++p) b[4 * j + p] = Bs[q][4 * (tx + 16 * j) + p];
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
     double *Y = wy + rows_off(nu, K) * (int64_t)NC;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        int gr = row0 + ty + 16 * i;
-        if (gr >= M) continue;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            int t = (col0 >> 2) + tx + 16 * j;
-            if (t >= C.ncols) continue;
-            double *y = Y + (int64_t)gr * NC + 4 * t;
-            y[0] = acc[i][4 * j + 0];
-            y[1] = acc[i][4 * j + 1];
-            y[2] = acc[i][4 * j + 2];
-            y[3] = acc[i][4 * j + 3];
+        for (int j = 0; j < 4; ++j) {
+            int gc = col0 + wc + 16 * j + li;
+            if (gc >= NC) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int gr = row0 + wr + 16 * i + 4 * v + lk;
+                if (gr < M) Y[(int64_t)gr * NC + gc] = acc[i][j][v];
+            }
         }
-    }
 }
 
 // Coaxial shift of one class: grid (translation tiles, row tiles, 2K+1 orders).
-// Z_k^a(n) = sum_nu A_k(n,nu) Y_k^a(nu) + B_k(n,nu) Y_k^b(nu);  Z_k^b = B Y^a + A Y^b
-// 64 rows x 64 translations per tile; thread (ty, tx) owns rows ty + 16 i and
-// translations tx + 16 j (i, j < 4).
+// Real form per (n, nu): [zar zai zbr zbi]^T = S(A_k(n,nu), B_k(n,nu)) [yar yai ybr ybi]^T with
+//   zar = Ar yar - Ai yai + Br ybr - Bi ybi     zai = Ai yar + Ar yai + Bi ybr + Br ybi
+//   zbr = Br yar - Bi yai + Ar ybr - Ai ybi     zbi = Bi yar + Br yai + Ai ybr + Ar ybi
+// i.e. entry (out, in) takes B when the halves differ, the imaginary part when the
+// parities differ, and a minus sign for (even out, odd in).
+// One 16x16x4 tile = 4 rows n x 4 output parts (tile row r: n = r % 4, part = r / 4),
+// 4 input parts of one nu (inner), 16 translations (columns), so that a lane ends up
+// holding the 4 parts of one (n, translation).  Block: 64 rows n x 64 translations,
+// waves split along n.
 constexpr int AM = 64;
 constexpr int AN = 64;
 constexpr int AK = 8;
@@ -151,104 +197,99 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
     const int NC = 4 * C.ncols;
     const double2 *Ab = wax + C.off_ax + ax_off(k, Lt, Ls, K);
     const double2 *Bb = Ab + ax_block(k, Lt, Ls);
-    __shared__ double2 As[AK][AM];
-    __shared__ double2 Bs2[AK][AM];
-    __shared__ double2 Ya[AK][AN];
-    __shared__ double2 Yb[AK][AN];
+    // As[c][r] holds A (x, y) and B (x, y) of row r at inner c as 4 consecutive doubles
+    __shared__ double As[AK][AM][4];
+    __shared__ double Ys[AK][4 * AN + 2];
     const int tid = threadIdx.x;
-    const int ty = tid >> 4, tx = tid & 15;
-    double2 za[4][4], zb[4][4];
+    const int lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nl = li & 3, cout = li >> 2;  // A-fragment row: (n within 4, output part)
+    const int sel = (((cout >> 1) ^ (lk >> 1)) & 1) * 2 + ((cout ^ lk) & 1);  // B? * 2 + imag?
+    const double sgn = ((cout & 1) == 0 && (lk & 1) == 1) ? -1.0 : 1.0;
+    dv4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            za[i][j] = make_double2(0.0, 0.0);
-            zb[i][j] = make_double2(0.0, 0.0);
-        }
-    for (int k0 = 0; k0 < KI; k0 += AK) {
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+    double2 pa[2], pb[2];
+    double py[8];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            int idx = tid + 256 * e;  // 0..511 over 64 rows x 8 inner
+            int idx = tid + 256 * e;  // 64 rows x 8 inner
             int r = idx >> 3, c = idx & 7;
             int gr = row0 + r, gc = k0 + c;
             bool ok = gr < M && gc < KI;
-            As[c][r] = ok ? Ab[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
-            Bs2[c][r] = ok ? Bb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
+            pa[e] = ok ? Ab[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
+            pb[e] = ok ? Bb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
         }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;  // 8 inner x 256 reals (64 translations x 4)
+            int c = idx >> 8, rr = idx & 255;
+            int nu = q + k0 + c;
+            int j = col0 + (rr >> 2);
+            double v = 0.0;
+            if (nu <= Ls && j < C.ncols) {
+                int kq = nu < K ? nu : K;
+                v = wy[(rows_off(nu, K) + (k + kq)) * (int64_t)NC + 4 * (int64_t)col0 + rr];
+            }
+            py[e] = v;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < KI; k0 += AK) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            int idx = tid + 256 * e;  // 0..511 over 8 inner x 64 translations
-            int c = idx >> 6, t = idx & 63;
-            int nu = q + k0 + c;
-            int j = col0 + t;
-            double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
-            if (nu <= Ls && j < C.ncols) {
-                int kk = nu < K ? nu : K;
-                const double *row = wy + (rows_off(nu, K) + (k + kk)) * (int64_t)NC + 4 * j;
-                va = make_double2(row[0], row[1]);
-                vb = make_double2(row[2], row[3]);
-            }
-            Ya[c][t] = va;
-            Yb[c][t] = vb;
+            int idx = tid + 256 * e;
+            int r = idx >> 3, c = idx & 7;
+            As[c][r][0] = pa[e].x;
+            As[c][r][1] = pa[e].y;
+            As[c][r][2] = pb[e].x;
+            As[c][r][3] = pb[e].y;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;
+            Ys[idx >> 8][idx & 255] = py[e];
         }
         __syncthreads();
+        if (k0 + AK < KI) load(k0 + AK);
 #pragma unroll
         for (int c = 0; c < AK; ++c) {
-            double2 a[4], b[4], ya[4], yb[4];
+            double a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = As[c][ty + 16 * i];
-                b[i] = Bs2[c][ty + 16 * i];
-            }
+            for (int i = 0; i < 4; ++i) a[i] = sgn * As[c][w * 16 + 4 * i + nl][sel];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                ya[j] = Ya[c][tx + 16 * j];
-                yb[j] = Yb[c][tx + 16 * j];
-            }
+            for (int j = 0; j < 4; ++j) b[j] = Ys[c][4 * (16 * j + li) + lk];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    za[i][j].x = fma(a[i].x, ya[j].x, za[i][j].x);
-                    za[i][j].x = fma(-a[i].y, ya[j].y, za[i][j].x);
-                    za[i][j].y = fma(a[i].x, ya[j].y, za[i][j].y);
-                    za[i][j].y = fma(a[i].y, ya[j].x, za[i][j].y);
-                    za[i][j].x = fma(b[i].x, yb[j].x, za[i][j].x);
-                    za[i][j].x = fma(-b[i].y, yb[j].y, za[i][j].x);
-                    za[i][j].y = fma(b[i].x, yb[j].y, za[i][j].y);
-                    za[i][j].y = fma(b[i].y, yb[j].x, za[i][j].y);
-                    zb[i][j].x = fma(b[i].x, ya[j].x, zb[i][j].x);
-                    zb[i][j].x = fma(-b[i].y, ya[j].y, zb[i][j].x);
-                    zb[i][j].y = fma(b[i].x, ya[j].y, zb[i][j].y);
-                    zb[i][j].y = fma(b[i].y, ya[j].x, zb[i][j].y);
-                    zb[i][j].x = fma(a[i].x, yb[j].x, zb[i][j].x);
-                    zb[i][j].x = fma(-a[i].y, yb[j].y, zb[i][j].x);
-                    zb[i][j].y = fma(a[i].x, yb[j].y, zb[i][j].y);
-                    zb[i][j].y = fma(a[i].y, yb[j].x, zb[i][j].y);
-                }
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
+    // lane holds, for translation 16 j + li, the parts v = 0..3 of row n = w*16 + 4 i + lk
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        int n = q + row0 + ty + 16 * i;
+        int n = q + row0 + w * 16 + 4 * i + lk;
         if (n > Lt) continue;
-        int kk = n < K ? n : K;
-        double *row = wz + (rows_off(n, K) + (k + kk)) * (int64_t)NC;
+        int kq = n < K ? n : K;
+        double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            int t = col0 + tx + 16 * j;
+            int t = col0 + 16 * j + li;
             if (t >= C.ncols) continue;
-            row[4 * t + 0] = za[i][j].x;
-            row[4 * t + 1] = za[i][j].y;
-            row[4 * t + 2] = zb[i][j].x;
-            row[4 * t + 3] = zb[i][j].y;
+            row[4 * t + 0] = acc[i][j][0];
+            row[4 * t + 1] = acc[i][j][1];
+            row[4 * t + 2] = acc[i][j][2];
+            row[4 * t + 3] = acc[i][j][3];
         }
     }
 }
 
 // Back rotation of one class with fused phase and scatter-add:
-// grid (col tiles, row tiles, degree-1); rows are m = -n..n, inner k.
+// grid (col tiles, row tiles, degree-1); rows m = -n..n, inner k.
 __global__ __launch_bounds__(256) void k_class_rot_back(const ClassDesc *__restrict__ cls,
                                                         const int64_t *__restrict__ dst_off,
                                                         const double *__restrict__ wd, const double *__restrict__ wz,
@@ -261,75 +302,98 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const ClassDesc *__restr
     const int M = 2 * n + 1;    // rows m
     const int KI = 2 * kk + 1;  // inner k
     const int NC = 4 * C.ncols;
-    const int row0 = blockIdx.y * BM;
-    const int col0 = blockIdx.x * BN;
+    const int row0 = blockIdx.y * RM;
+    const int col0 = blockIdx.x * RN;
     if (row0 >= M) return;
-    __shared__ double As[BK][BM];
-    __shared__ double Bs[BK][BN];
+    __shared__ double As[RK][RM + RPAD];
+    __shared__ double Bs[RK][RN + RPAD];
+    __shared__ double2 ph[RM];
+    for (int r = threadIdx.x; r < RM; r += blockDim.x) {
+        double sn, cs;
+        sincos(-(double)(row0 + r - n) * C.phi, &sn, &cs);
+        ph[r] = make_double2(cs, sn);
+    }
     const double *D = wd + C.off_d + wig_off(n, K);  // (KI x M) row-major: D[k][m]
     const double *Z = wz + rows_off(n, K) * (int64_t)NC;
     const int tid = threadIdx.x;
-    const int ty = tid >> 4, tx = tid & 15;
-    double acc[8][8];
+    const int lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+    dv4 acc[2][4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
-    for (int k0 = 0; k0 < KI; k0 += BK) {
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+    double ra[4], rb[8];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 8 inner x 128 rows, coalesced along m
-            int q = idx >> 7, r = idx & 127;
-            int gr = row0 + r, gq = k0 + q;
-            As[q][r] = (gr < M && gq < KI) ? D[(int64_t)gq * M + gr] : 0.0;
+            int idx = tid + 256 * e;  // 16 inner x 64 rows, coalesced along m
+            int qq = idx >> 6, r = idx & 63;
+            int gr = row0 + r, gq = k0 + qq;
+            ra[e] = (gr < M && gq < KI) ? D[(int64_t)gq * M + gr] : 0.0;
         }
 #pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;  // 16 inner x 128 columns
+            int qq = idx >> 7, cc = idx & 127;
+            int gq = k0 + qq, gc = col0 + cc;
+            rb[e] = (gq < KI && gc < NC) ? Z[(int64_t)gq * NC + gc] : 0.0;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < KI; k0 += RK) {
+#pragma unroll
         for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 8 inner x 128 columns
-            int q = idx >> 7, cc = idx & 127;
-            int gq = k0 + q, gc = col0 + cc;
-            Bs[q][cc] = (gq < KI && gc < NC) ? Z[(int64_t)gq * NC + gc] : 0.0;
+            int idx = tid + 256 * e;
+            As[idx >> 6][idx & 63] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;
+            Bs[idx >> 7][idx & 127] = rb[e];
         }
         __syncthreads();
+        if (k0 + RK < KI) load(k0 + RK);
 #pragma unroll
-        for (int q = 0; q < BK; ++q) {
-            double a[8], b[8];
+        for (int kc = 0; kc < RK; kc += 4) {
+            double a[2], b[4];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a[i] = As[q][ty + 16 * i];
+            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][wr + 16 * i + li];
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kc + lk][wc + 16 * j + li];
 #pragma unroll
-                for (int p = 0; p < 4; ++p) b[4 * j + p] = Bs[q][4 * (tx + 16 * j) + p];
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
+    // epilogue: the 4 parts of a translation sit in lanes li..li+3 (li % 4 == 0)
     const int64_t ndst = (int64_t)C.Lt * (C.Lt + 2);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        int t = (col0 >> 2) + tx + 16 * j;
-        if (t >= C.ncols) continue;
-        double2 *dst = dst_pool + dst_off[C.col_begin + t];
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            int gr = row0 + ty + 16 * i;
-            if (gr >= M) continue;
-            int m = gr - n;
-            double sn, cs;
-            sincos(-(double)m * C.phi, &sn, &cs);
-            double2 wa = make_double2(cs * acc[i][4 * j] - sn * acc[i][4 * j + 1],
-                                      cs * acc[i][4 * j + 1] + sn * acc[i][4 * j]);
-            double2 wb = make_double2(cs * acc[i][4 * j + 2] - sn * acc[i][4 * j + 3],
-                                      cs * acc[i][4 * j + 3] + sn * acc[i][4 * j + 2]);
-            int64_t o = fidx(n, m);
-            double2 da = dst[o], db = dst[o + ndst];
-            dst[o] = make_double2(da.x + wa.x, da.y + wa.y);
-            dst[o + ndst] = make_double2(db.x + wb.x, db.y + wb.y);
-        }
-    }
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double c0 = acc[i][j][v];
+                double c1 = __shfl_down(c0, 1, 64);
+                double c2 = __shfl_down(c0, 2, 64);
+                double c3 = __shfl_down(c0, 3, 64);
+                int gc = col0 + wc + 16 * j + li;
+                int lr = wr + 16 * i + 4 * v + lk;
+                int gr = row0 + lr;
+                if ((li & 3) != 0 || gc >= NC || gr >= M) continue;
+                int t = gc >> 2;
+                int m = gr - n;
+                double2 p = ph[lr];
+                double2 *dst = dst_pool + dst_off[C.col_begin + t];
+                int64_t o = fidx(n, m);
+                double2 da = dst[o], db = dst[o + ndst];
+                dst[o] = make_double2(da.x + p.x * c0 - p.y * c1, da.y + p.x * c1 + p.y * c0);
+                dst[o + ndst] = make_double2(db.x + p.x * c2 - p.y * c3, db.y + p.x * c3 + p.y * c2);
+            }
 }
 
 void launch_class(const ClassDesc &h, const ClassDesc *d_cls, const int64_t *src_off, const int64_t *dst_off,
@@ -338,16 +402,16 @@ void launch_class(const ClassDesc &h, const ClassDesc *d_cls, const int64_t *src
     const int NC = 4 * h.ncols;
     const int K = h.K;
     if (stage == 0) {
-        int mt = (2 * (h.Ls < K ? h.Ls : K) + 1 + BM - 1) / BM;
-        dim3 grid((NC + BN - 1) / BN, mt, h.Ls);
+        int mt = (2 * (h.Ls < K ? h.Ls : K) + 1 + RM - 1) / RM;
+        dim3 grid((NC + RN - 1) / RN, mt, h.Ls);
         hipLaunchKernelGGL(k_class_rot_fwd, grid, dim3(256), 0, s, d_cls, src_off, src_pool, wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
         dim3 grid((h.ncols + AN - 1) / AN, mt, 2 * K + 1);
         hipLaunchKernelGGL(k_class_axial, grid, dim3(256), 0, s, d_cls, wax, wy, wz);
     } else {
-        int mt = (2 * h.Lt + 1 + BM - 1) / BM;
-        dim3 grid((NC + BN - 1) / BN, mt, h.Lt);
+        int mt = (2 * h.Lt + 1 + RM - 1) / RM;
+        dim3 grid((NC + RN - 1) / RN, mt, h.Lt);
         hipLaunchKernelGGL(k_class_rot_back, grid, dim3(256), 0, s, d_cls, dst_off, wd, wz, dst_pool);
     }
 }
