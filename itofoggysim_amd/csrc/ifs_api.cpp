@@ -292,7 +292,7 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
         if (c->prof.on) {
             for (int n = 1; n <= T.Ls; ++n) ffwd += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
             for (int n = 1; n <= T.Lt; ++n) fback += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-            for (int k = -T.K; k <= T.K; ++k) fax += 32.0 * (double)ax_block(k, T.Lt, T.Ls);
+            for (int k = -T.K; k <= T.K; ++k) fax += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
             ffwd *= B.cls[i].ncols;
             fback *= B.cls[i].ncols;
             fax *= B.cls[i].ncols;
@@ -333,7 +333,7 @@ void class_batch_cost(const ClassBatch &B, double &flops, double &bytes) {
         double f = 0.0;
         for (int n = 1; n <= T.Ls; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
         for (int n = 1; n <= T.Lt; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-        for (int k = -T.K; k <= T.K; ++k) f += 32.0 * (double)ax_block(k, T.Lt, T.Ls);
+        for (int k = -T.K; k <= T.K; ++k) f += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
         flops += f * B.cls[i].ncols;
         bytes += (4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5) * B.cls[i].ncols;
     }
@@ -344,7 +344,7 @@ void batch_cost(const Batch &B, double &flops, double &bytes) {
         double f = 0.0;
         for (int n = 1; n <= T.Ls; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
         for (int n = 1; n <= T.Lt; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-        for (int k = -T.K; k <= T.K; ++k) f += 32.0 * (double)ax_block(k, T.Lt, T.Ls);
+        for (int k = -T.K; k <= T.K; ++k) f += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
         flops += f;
         bytes += 4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5;
     }

@@ -6,10 +6,12 @@
 // and every translation of the class becomes a column of three dense products:
 //
 //   forward rotation, degree nu:  Y^nu (rows k)  = D^nu (k x m) * X^nu (m x cols)
-//   coaxial shift, order k:       Z_k  (rows n)  = [A_k B_k] * [Y_k^a ; Y_k^b]   (complex, a/b mixed)
+//   coaxial shift, order k:       Z_k^+ = P_k Y_k^+,  Z_k^- = Q_k Y_k^-            (complex)
 //   back rotation, degree n:      W^n  (rows m)  = D^n^T (m x k) * Z^n (k x cols)
 //
-// Columns are the translations of the class, 4 reals each (a.re, a.im, b.re, b.im).
+// with x_+- = x_a +- x_b and P = A + B, Q = A - B (the [A B; B A] coaxial operator
+// decoupled).  Columns are the translations of the class, 4 reals each
+// (+.re, +.im, -.re, -.im).
 // The gather of the sources with the phase e^{i m phi} is fused into the loads of
 // the forward product, the phase e^{-i m phi} and the scatter-add into the targets
 // into the epilogue of the back product.  Targets of one class are distinct, so
@@ -129,7 +131,8 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restri
             int t = idx >> 4, q = idx & 15;
             int gq = k0 + q;
             double2 p = (gq < KI) ? ph[gq] : make_double2(1.0, 0.0);  // m + nu = gq
-            double2 xa = rxa[e], xb = rxb[e];
+            double2 xa = make_double2(rxa[e].x + rxb[e].x, rxa[e].y + rxb[e].y);  // x_+
+            double2 xb = make_double2(rxa[e].x - rxb[e].x, rxa[e].y - rxb[e].y);  // x_-
             Bs[q][4 * t + 0] = p.x * xa.x - p.y * xa.y;
             Bs[q][4 * t + 1] = p.x * xa.y + p.y * xa.x;
             Bs[q][4 * t + 2] = p.x * xb.x - p.y * xb.y;
@@ -167,15 +170,10 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restri
 }
 
 // Coaxial shift of one class: grid (translation tiles, row tiles, 2K+1 orders).
-// Real form per (n, nu): [zar zai zbr zbi]^T = S(A_k(n,nu), B_k(n,nu)) [yar yai ybr ybi]^T with
-//   zar = Ar yar - Ai yai + Br ybr - Bi ybi     zai = Ai yar + Ar yai + Bi ybr + Br ybi
-//   zbr = Br yar - Bi yai + Ar ybr - Ai ybi     zbi = Bi yar + Br yai + Ai ybr + Ar ybi
-// i.e. entry (out, in) takes B when the halves differ, the imaginary part when the
-// parities differ, and a minus sign for (even out, odd in).
-// One 16x16x4 tile = 4 rows n x 4 output parts (tile row r: n = r % 4, part = r / 4),
-// 4 input parts of one nu (inner), 16 translations (columns), so that a lane ends up
-// holding the 4 parts of one (n, translation).  Block: 64 rows n x 64 translations,
-// waves split along n.
+//   Z^+ = P Y^+,  Z^- = Q Y^-  (complex), computed as real products of
+//   [[Pr, -Pi], [Pi, Pr]] with [Yr; Yi].  One 16x16x4 tile = 8 rows n x 2 parts
+//   (tile row r: n = r % 8, part = r / 8), inner = 2 nu x 2 parts, 16 translations.
+//   Block: 64 rows n x 64 translations; wave w owns rows 16 w .. 16 w + 15.
 constexpr int AM = 64;
 constexpr int AN = 64;
 constexpr int AK = 8;
@@ -195,22 +193,26 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
     const int col0 = blockIdx.x * AN;
     if (row0 >= M) return;
     const int NC = 4 * C.ncols;
-    const double2 *Ab = wax + C.off_ax + ax_off(k, Lt, Ls, K);
-    const double2 *Bb = Ab + ax_block(k, Lt, Ls);
-    // As[c][r] holds A (x, y) and B (x, y) of row r at inner c as 4 consecutive doubles
-    __shared__ double As[AK][AM][4];
+    const double2 *Pb = wax + C.off_ax + ax_off(k, Lt, Ls, K);
+    const double2 *Qb = Pb + ax_block(k, Lt, Ls);
+    __shared__ double2 Ps[AK][AM];
+    __shared__ double2 Qs[AK][AM];
     __shared__ double Ys[AK][4 * AN + 2];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int nl = li & 3, cout = li >> 2;  // A-fragment row: (n within 4, output part)
-    const int sel = (((cout >> 1) ^ (lk >> 1)) & 1) * 2 + ((cout ^ lk) & 1);  // B? * 2 + imag?
-    const double sgn = ((cout & 1) == 0 && (lk & 1) == 1) ? -1.0 : 1.0;
-    dv4 acc[4][4];
+    const int nl = li & 7, pout = li >> 3;   // A-fragment row: (n within 8, output part)
+    const int nul = lk >> 1, pin = lk & 1;   // inner: (nu within 2, input part)
+    const bool use_im = pout != pin;
+    const double sgn = (pout == 0 && pin == 1) ? -1.0 : 1.0;
+    dv4 accP[2][4], accQ[2][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j) {
+            accP[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+            accQ[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        }
     double2 pa[2], pb[2];
     double py[8];
     auto load = [&](int k0) {
@@ -220,8 +222,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
             int r = idx >> 3, c = idx & 7;
             int gr = row0 + r, gc = k0 + c;
             bool ok = gr < M && gc < KI;
-            pa[e] = ok ? Ab[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
-            pb[e] = ok ? Bb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
+            pa[e] = ok ? Pb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
+            pb[e] = ok ? Qb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -242,11 +244,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             int idx = tid + 256 * e;
-            int r = idx >> 3, c = idx & 7;
-            As[c][r][0] = pa[e].x;
-            As[c][r][1] = pa[e].y;
-            As[c][r][2] = pb[e].x;
-            As[c][r][3] = pb[e].y;
+            Ps[idx & 7][idx >> 3] = pa[e];
+            Qs[idx & 7][idx >> 3] = pb[e];
         }
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -256,36 +255,51 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
         __syncthreads();
         if (k0 + AK < KI) load(k0 + AK);
 #pragma unroll
-        for (int c = 0; c < AK; ++c) {
-            double a[4], b[4];
+        for (int cc = 0; cc < AK; cc += 2) {
+            double aP[2], aQ[2], bP[4], bQ[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = sgn * As[c][w * 16 + 4 * i + nl][sel];
+            for (int i = 0; i < 2; ++i) {
+                int r = w * 16 + 8 * i + nl;
+                double2 p = Ps[cc + nul][r], qv = Qs[cc + nul][r];
+                aP[i] = sgn * (use_im ? p.y : p.x);
+                aQ[i] = sgn * (use_im ? qv.y : qv.x);
+            }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Ys[c][4 * (16 * j + li) + lk];
+            for (int j = 0; j < 4; ++j) {
+                int col = 16 * j + li;
+                bP[j] = Ys[cc + nul][4 * col + pin];
+                bQ[j] = Ys[cc + nul][4 * col + 2 + pin];
+            }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; ++j) {
+                    accP[i][j] = mfma(aP[i], bP[j], accP[i][j]);
+                    accQ[i][j] = mfma(aQ[i], bQ[j], accQ[i][j]);
+                }
         }
         __syncthreads();
     }
-    // lane holds, for translation 16 j + li, the parts v = 0..3 of row n = w*16 + 4 i + lk
+    // D row 4 v + lk of tile i: n = 8 i + ((4 v + lk) & 7), part = (4 v + lk) >> 3, i.e.
+    // v = 0, 2 -> (re, im) of n = 8 i + lk;  v = 1, 3 -> (re, im) of n = 8 i + 4 + lk
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        int n = q + row0 + w * 16 + 4 * i + lk;
-        if (n > Lt) continue;
-        int kq = n < K ? n : K;
-        double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            int t = col0 + 16 * j + li;
-            if (t >= C.ncols) continue;
-            row[4 * t + 0] = acc[i][j][0];
-            row[4 * t + 1] = acc[i][j][1];
-            row[4 * t + 2] = acc[i][j][2];
-            row[4 * t + 3] = acc[i][j][3];
+        for (int h = 0; h < 2; ++h) {
+            int n = q + row0 + w * 16 + 8 * i + 4 * h + lk;
+            if (n > Lt) continue;
+            int kq = n < K ? n : K;
+            double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int t = col0 + 16 * j + li;
+                if (t >= C.ncols) continue;
+                row[4 * t + 0] = accP[i][j][h];
+                row[4 * t + 1] = accP[i][j][h + 2];
+                row[4 * t + 2] = accQ[i][j][h];
+                row[4 * t + 3] = accQ[i][j][h + 2];
+            }
         }
-    }
 }
 
 // Back rotation of one class with fused phase and scatter-add:
@@ -390,9 +404,12 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const ClassDesc *__restr
                 double2 p = ph[lr];
                 double2 *dst = dst_pool + dst_off[C.col_begin + t];
                 int64_t o = fidx(n, m);
+                // (c0, c1) = back-rotated z_+, (c2, c3) = z_-: a = (z_+ + z_-)/2, b = (z_+ - z_-)/2
+                double ar = 0.5 * (c0 + c2), ai = 0.5 * (c1 + c3);
+                double br = 0.5 * (c0 - c2), bi = 0.5 * (c1 - c3);
                 double2 da = dst[o], db = dst[o + ndst];
-                dst[o] = make_double2(da.x + p.x * c0 - p.y * c1, da.y + p.x * c1 + p.y * c0);
-                dst[o + ndst] = make_double2(db.x + p.x * c2 - p.y * c3, db.y + p.x * c3 + p.y * c2);
+                dst[o] = make_double2(da.x + p.x * ar - p.y * ai, da.y + p.x * ai + p.y * ar);
+                dst[o + ndst] = make_double2(db.x + p.x * br - p.y * bi, db.y + p.x * bi + p.y * br);
             }
 }
 

@@ -4,10 +4,11 @@
 //   gen_wigner : Wigner d^n_{km}(theta) tables by the three-term recurrence in n
 //   gen_sect   : sectorial scalar coaxial coefficients (recurrence in m)
 //   gen_axial  : vector coaxial coefficients A_k(n,nu), B_k(n,nu) (recurrence in nu)
-//   rot_fwd    : y = d(theta) diag(e^{i m phi}) x        (forward rotation)
-//   axial_apply: z = [A B; B A] y per coaxial order k
-//   rot_back   : target += diag(e^{-i m phi}) d(theta)^T z, summed over the tasks of
-//                one target in a fixed order (deterministic, no atomics)
+//   rot_fwd    : y_+- = d(theta) diag(e^{i m phi}) (x_a +- x_b)   (forward rotation)
+//   axial_apply: z_+ = P_k y_+, z_- = Q_k y_- per coaxial order k, P = A + B, Q = A - B
+//                (the [A B; B A] coaxial operator decoupled)
+//   rot_back   : target_a/b += diag(e^{-i m phi}) d(theta)^T (z_+ +- z_-)/2, summed over
+//                the tasks of one target in a fixed order (deterministic, no atomics)
 // The algorithm replaces the Gaunt/3j evaluation of Xu's A/B of
 // libmath/src/math/lib_math_vector_spherical_harmonics.f90:1354-1585 by the
 // rotation / coaxial-shift factorisation; both produce the same operator.
@@ -249,7 +250,8 @@ void launch_gen_sect(const Task *tasks, int ntasks, int max_nmax, double2 *wsect
 //   A_m(n,nu) = i^{nu-n} [sqrt(n(n+1)) al(n,nu) + t a_n sqrt(n/(n+1)) al(n+1,nu)
 //                         + t a_{n-1} sqrt((n+1)/n) al(n-1,nu)] / sqrt(nu(nu+1))
 //   B_m(n,nu) = i^{nu-n+1} t m al(n,nu) / sqrt(n(n+1) nu(nu+1))
-// Order -m shares A and has B negated.
+// Order -m shares A and has B negated.  The table stores P = A + B and Q = A - B:
+// with y_+- = y_a +- y_b the shift decouples into z_+ = P y_+ and z_- = Q y_-.
 // ------------------------------------------------------------------------
 __global__ void k_gen_axial(const Task *__restrict__ tasks, int max_nmax, const double2 *__restrict__ wsect,
                             double2 *__restrict__ wax) {
@@ -291,11 +293,12 @@ __global__ void k_gen_axial(const Task *__restrict__ tasks, int max_nmax, const 
                 double2 av = ipow_mul((nu - n) & 3, cscale(inv_nu, s));
                 double2 bv = ipow_mul((nu - n + 1) & 3, cscale(t * (double)m * inv_nu / sqrt(nn * (nn + 1.0)), c0));
                 int64_t o = (int64_t)(n - q) * ncol + (nu - q);
-                Ap[o] = av;
-                Bp[o] = bv;
-                if (m > 0) {
-                    An[o] = av;
-                    Bn[o] = make_double2(-bv.x, -bv.y);
+                double2 pv = cadd(av, bv), qv = csub(av, bv);
+                Ap[o] = pv;  // P_m = A_m + B_m
+                Bp[o] = qv;  // Q_m = A_m - B_m
+                if (m > 0) {  // order -m: A unchanged, B negated
+                    An[o] = qv;
+                    Bn[o] = pv;
                 }
             }
         }
@@ -374,8 +377,8 @@ __global__ void k_rot_fwd(const Task *__restrict__ tasks, const double2 *__restr
         sb.y += d * vb.y;
     }
     double2 *y = wy + T.off_y;
-    y[e] = sa;
-    y[e + nent] = sb;
+    y[e] = cadd(sa, sb);       // y_+
+    y[e + nent] = csub(sa, sb);  // y_-
 }
 
 void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls, const double2 *src_pool,
@@ -387,8 +390,8 @@ void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls,
 }
 
 // ------------------------------------------------------------------------
-// Coaxial shift: z(n,k) = sum_nu A_k(n,nu) y_a(nu,k) + B_k(n,nu) y_b(nu,k)  (a-part)
-//                z_b    = sum_nu B_k(n,nu) y_a + A_k(n,nu) y_b
+// Coaxial shift in the decoupled form: z_+(n,k) = sum_nu P_k(n,nu) y_+(nu,k),
+//                                     z_-(n,k) = sum_nu Q_k(n,nu) y_-(nu,k)
 // ------------------------------------------------------------------------
 __global__ void k_axial_apply(const Task *__restrict__ tasks, const double2 *__restrict__ wax,
                               const double2 *__restrict__ wy, double2 *__restrict__ wz) {
@@ -421,10 +424,8 @@ __global__ void k_axial_apply(const Task *__restrict__ tasks, const double2 *__r
     for (int nu = q; nu <= Ls; ++nu) {
         int kk = nu < K ? nu : K;
         int64_t iy = rows_off(nu, K) + k + kk;
-        double2 a = Ab[nu - q], b = Bb[nu - q];
-        double2 va = ya[iy], vb = yb[iy];
-        za = cadd(za, cadd(cmul(a, va), cmul(b, vb)));
-        zb = cadd(zb, cadd(cmul(b, va), cmul(a, vb)));
+        za = cadd(za, cmul(Ab[nu - q], ya[iy]));
+        zb = cadd(zb, cmul(Bb[nu - q], yb[iy]));
     }
     double2 *z = wz + T.off_z;
     z[e] = za;
@@ -476,9 +477,10 @@ __global__ void k_rot_back(const Task *__restrict__ tasks, const Segment *__rest
             oa = cadd(oa, cmul(e_, sa));
             ob = cadd(ob, cmul(e_, sb));
         }
+        // oa, ob hold the back-rotated z_+, z_-: a = (z_+ + z_-)/2, b = (z_+ - z_-)/2
         double2 *dst = dst_pool + S.dst;
-        dst[e] = cadd(dst[e], oa);
-        dst[e + nent] = cadd(dst[e + nent], ob);
+        dst[e] = cadd(dst[e], cscale(0.5, cadd(oa, ob)));
+        dst[e + nent] = cadd(dst[e + nent], cscale(0.5, csub(oa, ob)));
     }
 }
 
