@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also run GMRES(50) to 1e-6")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); run with --gpus 1",
@@ -61,6 +63,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "dispatches": 3}))
+        return
     from itofoggysim_amd import MultiSphereSolver, config_cluster
 
     c = config_cluster(2, n=args.n if args.n != 2000 else None)

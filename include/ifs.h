@@ -44,6 +44,15 @@ typedef struct {
 #define IFS_ZSEL_BESSEL_J 1
 #define IFS_ZSEL_HANKEL1 3
 
+/* Translation vector of the sphere/box couplings:
+ *   IFS_D_PHYSICAL  d = origin(target) - origin(source): the sign for which Xu's
+ *                   coefficients as the reference evaluates them conserve energy
+ *                   (lossless clusters: far-field integral = extinction);
+ *   IFS_D_REFERENCE d = origin(source) - origin(target), as the reference's direct
+ *                   path passes it (lib_mie_ms_solver_interface.f90:941-943). */
+#define IFS_D_PHYSICAL 0
+#define IFS_D_REFERENCE 1
+
 /* Parameters.  Replaces simulation_data%spherical_harmonics and the solver
  * parameter type (lib_mie_type.f90:84-99, lib_mie_ms_solver_interface.f90:91-129). */
 typedef struct {
@@ -55,7 +64,10 @@ typedef struct {
     int32_t leaf_size;       /* MLFMM: max spheres per leaf box (reference tree_s_opt) */
     double fmm_d0;           /* MLFMM: box truncation L = kD + 1.8 d0^(2/3) (kD)^(1/3); D = box edge */
     int32_t fmm_lmax_cap;    /* MLFMM: upper bound on box truncation (0: none) */
-    int32_t reserved[4];
+    int32_t workspace_mb;    /* translation workspace limit in MiB (0: min(free/4, 48 GiB)); small
+                                values split the translation classes into more workspace chunks */
+    int32_t d_convention;    /* IFS_D_PHYSICAL (default) or IFS_D_REFERENCE */
+    int32_t reserved[2];
 } ifs_params;
 
 /* Builds device tables, the Mie T-matrix and (MLFMM) the octree.
@@ -75,6 +87,14 @@ int ifs_matvec(ifs_ctx *ctx, const double *x, double *b, int64_t n);
 /* Same, on device buffers already resident in HBM, enqueued on the context stream. */
 int ifs_matvec_device(ifs_ctx *ctx, const void *x_dev, void *b_dev, int64_t n);
 
+/* b_r = (I + T A) x_r for nrhs right-hand sides in one pass: every translation table is
+ * generated once and applied to all of them (the translations become nrhs columns each
+ * of the class products).  Device buffers holding the vectors at stride ld >= n
+ * (complex); no reference counterpart (the reference applies one vector per call). */
+int ifs_matvec_batch_device(ifs_ctx *ctx, int32_t nrhs, const void *x_dev, void *b_dev, int64_t n, int64_t ld);
+/* Same on host buffers [nrhs][n]. */
+int ifs_matvec_batch(ifs_ctx *ctx, int32_t nrhs, const double *x, double *b, int64_t n);
+
 /* Launch stream: NULL restores the context's own stream.  The stream must be a
  * default-priority HIP stream. */
 int ifs_set_stream(ifs_ctx *ctx, void *hip_stream);
@@ -87,6 +107,39 @@ int ifs_get_mie(ifs_ctx *ctx, double *an, double *bn);
 /* Right-hand side for one x-polarised plane wave travelling along +z:
  * b = T p (p_nm, q_nm of lib_mie_illumination.f90:636-763 at alpha = beta = 0). */
 int ifs_plane_wave_rhs(ifs_ctx *ctx, double *rhs, int64_t n);
+
+/* Incident plane wave of unit amplitude travelling along (alpha, beta) (polar and
+ * azimuthal angle of the propagation direction; at alpha = 0 the field is polarised
+ * along (cos beta, sin beta, 0)): p_mn, q_mn at every sphere centre in the flat layout,
+ *   p_mn = e^{i k khat.d_j} tau_mn(alpha) e^{-i m beta} / (n(n+1)), q_mn with pi_mn
+ * (lib_mie_illumination.f90:636-763, get_p_q_j_j_plane_wave_core).  Host buffers. */
+int ifs_plane_wave(ifs_ctx *ctx, double alpha, double beta, double *p, int64_t n);
+/* rhs = T p for that wave; rows beyond a sphere's L_j (and of inactive spheres) are 0.
+ * Generalises ifs_plane_wave_rhs (alpha = beta = 0). */
+int ifs_plane_wave_rhs_dir(ifs_ctx *ctx, double alpha, double beta, double *rhs, int64_t n);
+
+/* Cluster cross sections from the solved coefficients x and the incident p of the same
+ * wave (flat layout): out = {C_sca, C_ext, C_abs} in the squared length unit of 1/k.
+ * Per-sphere sums of lib_mie_ss_helper_functions.f90:1380-1460 (Xu eq. 57) for the
+ * extinction; the absorption of each sphere from its exciting field a^j_mn / a_n^j;
+ * C_sca = C_ext - C_abs. */
+int ifs_cross_sections(ifs_ctx *ctx, const double *x, const double *p, int64_t n, double out[3]);
+int ifs_cross_sections_device(ifs_ctx *ctx, const void *x_dev, const void *p_dev, int64_t n, double out[3]);
+
+/* Far field of the scattered wave, E_s ~ e^{ikr}/(kr) (F_theta theta_hat + F_phi phi_hat),
+ * at ndir directions (theta[i], phi[i]) from the solved coefficients x (flat layout):
+ * out[4i..4i+3] = Re F_theta, Im F_theta, Re F_phi, Im F_phi.  No reference counterpart
+ * for clusters (single sphere: the Barber-Hill S2 program, light_scattering.f90:54-204). */
+int ifs_far_field(ifs_ctx *ctx, const double *x, int64_t n, int32_t ndir, const double *theta, const double *phi,
+                  double *out);
+int ifs_far_field_device(ifs_ctx *ctx, const void *x_dev, int64_t n, int32_t ndir, const void *theta_dev,
+                         const void *phi_dev, void *out_dev);
+/* Amplitude matrix S1..S4 (Bohren & Huffman 3.12, incidence along +z) from the far
+ * fields Fx, Fy of the x- (beta = 0) and y-polarised (beta = pi/2) waves:
+ * S[8i..8i+7] = S1, S2, S3, S4 (re, im). */
+int ifs_amplitude_matrix(int32_t ndir, const double *phi, const double *Fx, const double *Fy, double *S);
+/* Mueller matrix (Bohren & Huffman 3.16): M[16i + 4r + c] = S_{r+1,c+1}. */
+int ifs_mueller(int32_t ndir, const double *S, double *M);
 
 typedef struct {
     int32_t restart;      /* Krylov subspace size (reference default 10; BASELINE uses 50) */
@@ -111,6 +164,13 @@ int ifs_solve(ifs_ctx *ctx, const double *rhs, double *x, int64_t n, const ifs_s
 /* Same on device buffers. */
 int ifs_solve_device(ifs_ctx *ctx, const void *rhs_dev, void *x_dev, int64_t n, const ifs_solve_opts *opts,
                      ifs_solve_info *info);
+/* nrhs systems (I + T A) x_r = rhs_r by restarted GMRES in lock step: one batched
+ * matvec per Arnoldi step for all right-hand sides, per-RHS orthogonalisation and
+ * convergence (info[nrhs]).  Device buffers at stride ld; host variant [nrhs][n]. */
+int ifs_solve_batch_device(ifs_ctx *ctx, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n, int64_t ld,
+                           const ifs_solve_opts *opts, ifs_solve_info *info);
+int ifs_solve_batch(ifs_ctx *ctx, int32_t nrhs, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
+                    ifs_solve_info *info);
 
 /* Octree export for bit-exact checks (lib_tree.f90:147-270, lib_ml_fmm_helper_functions.f90:366-446).
  *   uindex_out[n_spheres]: Morton index of each sphere's leaf box at level l_max
@@ -130,6 +190,46 @@ int ifs_tree_neighbours(int64_t k, int64_t box, int32_t level, int64_t out[26]);
 int ifs_tree_centre(int64_t box, int32_t level, double out[3]);
 int ifs_tree_domain_i4(int64_t box, int32_t level, int64_t *out, int32_t cap);
 int ifs_tree_level_max(const int64_t *sorted_uindex, int64_t n, int64_t s);
+
+/* Octree build on the GPU, bit-exact with the host functions above (same scaling,
+ * Morton index, s-rule): the universal indices of the n points at the threshold
+ * level 21, sorted ascending (uindex_sorted[n]), the permutation that sorts them
+ * (perm[n], stable), levels = {l_min, l_max}.  Replaces lib_tree_constructor's
+ * sort and level search (lib_tree.f90:147-270, 1162-1242).  Host buffers. */
+int ifs_tree_build_gpu(const double *pts, int32_t n, int32_t s, int64_t *uindex_sorted, int32_t *perm,
+                       int32_t levels[2]);
+/* I4 (k = 1) of every box of an ascending list of distinct boxes at one level,
+ * restricted to the boxes of the list, ascending: out[nb * 189] (-1 padded),
+ * count[nb] (lib_tree.f90:994-1143).  GPU; host buffers. */
+int ifs_tree_i4_gpu(const int64_t *boxes, int64_t nb, int32_t level, int64_t *out, int32_t *count);
+
+/* Serving: continuous batching of scattering requests on one GPU.  A request is a
+ * cluster with its parameters and ndir incidence directions (alpha[i], beta[i]); its
+ * result is {C_sca, C_ext, C_abs} per direction, solved to the relative residual tol
+ * with lock-step GMRES(restart).  `slots` worker threads each run one request at a
+ * time on a context with its own stream, so admitted requests overlap on the device;
+ * a queued request is admitted when a slot is free and its estimated HBM footprint
+ * fits what the running ones leave of hbm_budget_gb (or nothing else runs).  Inputs
+ * are copied at submit.  No reference counterpart (the reference solves one cluster
+ * per process). */
+typedef struct ifs_server ifs_server;
+typedef struct {
+    int32_t status;          /* IFS_OK or the request's error */
+    int32_t iterations;      /* largest matvec count over the directions */
+    double latency_s;        /* submit -> done */
+    double queue_s;          /* submit -> admitted */
+    int64_t footprint_bytes; /* estimate used for admission */
+} ifs_request_info;
+int ifs_server_create(ifs_server **srv, int32_t slots, double hbm_budget_gb);
+int ifs_server_submit(ifs_server *srv, const ifs_cluster *cluster, const ifs_params *params, int32_t ndir,
+                      const double *alpha, const double *beta, double tol, int32_t restart, int64_t *request_id);
+/* Blocks until the request is done; csec[3 * ndir]; returns the request's status. */
+int ifs_server_wait(ifs_server *srv, int64_t request_id, double *csec, ifs_request_info *info);
+/* out = {completed, p50 latency s, p99 latency s, completed per second since create,
+ *        peak concurrent requests, peak admitted footprint bytes}. */
+int ifs_server_stats(ifs_server *srv, double out[6]);
+/* Finishes the queued requests, then stops the workers. */
+int ifs_server_destroy(ifs_server *srv);
 
 /* Interaction counts of the MLFMM plan per kind (P2M, M2M, M2L, L2L, L2P, P2P)
  * and the box truncation per level (levels_L[l] for l = 0..31, 0 if unused). */

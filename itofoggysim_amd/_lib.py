@@ -34,12 +34,18 @@ class Cluster(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("k", ctypes.c_double), ("n_global", ctypes.c_int32), ("z_selector", ctypes.c_int32),
                 ("ref_row_zeroing", ctypes.c_int32), ("use_fmm", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
-                ("fmm_d0", ctypes.c_double), ("fmm_lmax_cap", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("fmm_d0", ctypes.c_double), ("fmm_lmax_cap", ctypes.c_int32), ("workspace_mb", ctypes.c_int32),
+                ("d_convention", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("restart", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
                 ("use_x0", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32), ("latency_s", ctypes.c_double),
+                ("queue_s", ctypes.c_double), ("footprint_bytes", ctypes.c_int64)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -75,6 +81,27 @@ SIGNATURES = {
     "ifs_profile_enable": (_i32, [_vp, _i32]),
     "ifs_profile_read": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32]),
     "ifs_matvec_cost": (_i32, [_vp, _vp, _vp]),
+    "ifs_plane_wave": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64]),
+    "ifs_plane_wave_rhs_dir": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64]),
+    "ifs_cross_sections": (_i32, [_vp, _vp, _vp, _i64, _vp]),
+    "ifs_cross_sections_device": (_i32, [_vp, _vp, _vp, _i64, _vp]),
+    "ifs_far_field": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "ifs_far_field_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "ifs_amplitude_matrix": (_i32, [_i32, _vp, _vp, _vp, _vp]),
+    "ifs_mueller": (_i32, [_i32, _vp, _vp]),
+    "ifs_matvec_batch_device": (_i32, [_vp, _i32, _vp, _vp, _i64, _i64]),
+    "ifs_matvec_batch": (_i32, [_vp, _i32, _vp, _vp, _i64]),
+    "ifs_solve_batch_device": (_i32, [_vp, _i32, _vp, _vp, _i64, _i64, ctypes.POINTER(SolveOpts),
+                                      ctypes.POINTER(SolveInfo)]),
+    "ifs_solve_batch": (_i32, [_vp, _i32, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "ifs_server_create": (_i32, [ctypes.POINTER(_vp), _i32, ctypes.c_double]),
+    "ifs_server_submit": (_i32, [_vp, ctypes.POINTER(Cluster), ctypes.POINTER(Params), _i32, _vp, _vp,
+                                 ctypes.c_double, _i32, ctypes.POINTER(_i64)]),
+    "ifs_server_wait": (_i32, [_vp, _i64, _vp, ctypes.POINTER(RequestInfo)]),
+    "ifs_server_stats": (_i32, [_vp, _vp]),
+    "ifs_server_destroy": (_i32, [_vp]),
+    "ifs_tree_build_gpu": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "ifs_tree_i4_gpu": (_i32, [_vp, _i64, _i32, _vp, _vp]),
 }
 
 _lib = None

@@ -68,10 +68,33 @@ Task ifs::make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_
 
 namespace ifs {
 
+// Profiler names per batch kind (static strings).
+static void set_prof_names(Batch &B) {
+    static const char *p2m[6] = {"p2m_gen_wigner", "p2m_gen_sect", "p2m_gen_axial", "p2m_rot_fwd", "p2m_axial", "p2m_rot_back"};
+    static const char *l2p[6] = {"l2p_gen_wigner", "l2p_gen_sect", "l2p_gen_axial", "l2p_rot_fwd", "l2p_axial", "l2p_rot_back"};
+    static const char *p2p[6] = {"p2p_gen_wigner", "p2p_gen_sect", "p2p_gen_axial", "p2p_rot_fwd", "p2p_axial", "p2p_rot_back"};
+    const char **src = nullptr;
+    if (B.name == "p2m") src = p2m;
+    if (B.name == "l2p") src = l2p;
+    if (B.name == "p2p") src = p2p;
+    if (src)
+        for (int i = 0; i < 6; ++i) B.pn[i] = src[i];
+}
+
+// Algorithmic flops of one generic task: rotation forward, coaxial shift, rotation back.
+static void task_flops(const Task &T, double &ffwd, double &fax, double &fback) {
+    ffwd = fax = fback = 0.0;
+    for (int n = 1; n <= T.Ls; ++n) ffwd += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+    for (int n = 1; n <= T.Lt; ++n) fback += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+    for (int k = -T.K; k <= T.K; ++k) fax += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
+}
+
 // Split a target-sorted task list into segments and workspace chunks and upload it.
+// The tables are kept across matvecs when they fit the context's cache budget.
 int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) {
     B.segs.clear();
     B.chunks.clear();
+    set_prof_names(B);
     const size_t n = B.tasks.size();
     // per-task sizes
     auto sizes = [](const Task &T, int64_t &nd, int64_t &nsct, int64_t &nax, int64_t &ny, int64_t &nz) {
@@ -88,7 +111,7 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         sizes(T, nd, nsct, nax, ny, nz);
         tot_bytes += nd * 8 + (nsct + nax + ny + nz) * 16;
     }
-    B.cached = allow_cache && (size_t)tot_bytes + c->ws_cached <= budget_bytes;
+    B.cached = allow_cache && (size_t)tot_bytes + c->ws_cached <= c->cache_budget;
     size_t chunk_budget = B.cached ? (size_t)tot_bytes + 1 : budget_bytes;
 
     Chunk cur;
@@ -144,6 +167,11 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         cur.max_rows_z = std::max(cur.max_rows_z, rows_size(T.Lt, T.K));
         int nent = T.Lt * (T.Lt + 2);
         cur.rowback_blocks = std::max(cur.rowback_blocks, std::min(64, (nent + 255) / 256));
+        double f1, f2, f3;
+        task_flops(T, f1, f2, f3);
+        cur.f_fwd += f1;
+        cur.f_ax += f2;
+        cur.f_back += f3;
         used += bytes;
     }
     if (n > 0) close_chunk((int)n);
@@ -193,7 +221,8 @@ Scope::~Scope() {
     }
 }
 
-int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) {
+int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, int nrhs, int64_t src_stride,
+              int64_t dst_stride) {
     const Task *dt = B.d_tasks.as<Task>();
     double *wd = B.cached ? B.own_d.as<double>() : c->ws_d.as<double>();
     double2 *wsc = B.cached ? B.own_s.as<double2>() : c->ws_s.as<double2>();
@@ -208,29 +237,34 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) 
         const Task *tt = dt + ch.t_begin;
         if (!(B.cached && B.tables_ready)) {
             {
-                Scope s(c, "gen_wigner");
+                Scope s(c, B.pn[0]);
                 launch_gen_wigner(tt, nt, ch.maxpairs, wd, c->stream);
             }
             {
-                Scope s(c, "gen_sect");
+                Scope s(c, B.pn[1]);
                 launch_gen_sect(tt, nt, ch.max_nmax, wsc, c->stream);
             }
             {
-                Scope s(c, "gen_axial");
+                Scope s(c, B.pn[2]);
                 launch_gen_axial(tt, nt, ch.maxK, ch.max_nmax, wsc, wax, c->stream);
             }
         }
-        {
-            Scope s(c, "rot_fwd");
-            launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, src_pool, wd, wy, c->stream);
-        }
-        {
-            Scope s(c, "axial_apply");
-            launch_axial_apply(tt, nt, ch.max_rows_z, wax, wy, wz, c->stream);
-        }
-        {
-            Scope s(c, "rot_back");
-            launch_rot_back(dt, dsg + ch.s_begin, nsg, ch.rowback_blocks, wd, wz, dst_pool, c->stream);
+        // the tables serve every right-hand side
+        for (int r = 0; r < nrhs; ++r) {
+            const double2 *sp = src_pool + r * src_stride;
+            double2 *dp = dst_pool + r * dst_stride;
+            {
+                Scope s(c, B.pn[3], ch.f_fwd);
+                launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, sp, wd, wy, c->stream);
+            }
+            {
+                Scope s(c, B.pn[4], ch.f_ax);
+                launch_axial_apply(tt, nt, ch.max_rows_z, wax, wy, wz, c->stream);
+            }
+            {
+                Scope s(c, B.pn[5], ch.f_back);
+                launch_rot_back(dt, dsg + ch.s_begin, nsg, ch.rowback_blocks, wd, wz, dp, c->stream);
+            }
         }
         if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
     }
@@ -238,25 +272,182 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool) 
     return IFS_OK;
 }
 
+// Per-column algorithmic flops of the three products (folded rotations, decoupled
+// coaxial shift): forward rotation, coaxial shift, back rotation.
+static void class_col_flops(const Task &T, double &ffwd, double &fax, double &fback) {
+    ffwd = fax = fback = 0.0;
+    // folded rotation: E (kk+1)x(n+1) and O kk x n (its zero row and column excluded)
+    auto rot = [&](int n) {
+        double kk = std::min(n, T.K);
+        return 8.0 * ((kk + 1.0) * (n + 1.0) + kk * n);
+    };
+    for (int n = 1; n <= T.Ls; ++n) ffwd += rot(n);
+    for (int n = 1; n <= T.Lt; ++n) fback += rot(n);
+    for (int k = -T.K; k <= T.K; ++k) fax += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
+}
+
+static bool same_d(const Task &a, const Task &b) { return a.cth == b.cth && a.Ls == b.Ls && a.Lt == b.Lt && a.K == b.K; }
+static bool same_ax(const Task &a, const Task &b) {
+    return a.t == b.t && a.kind == b.kind && a.Ls == b.Ls && a.Lt == b.Lt && a.K == b.K;
+}
+
 int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
-    const size_t nc = B.cls.size();
-    B.regen_d.assign(nc, 1);
-    B.regen_ax.assign(nc, 1);
-    for (size_t i = 0; i < nc; ++i) {
-        const Task &T = B.gen[i];
-        int L = std::max(T.Ls, T.Lt);
-        int nmax = T.Lt + T.Ls + 2;
-        B.n_d = std::max<int64_t>(B.n_d, wig_size(L, T.K));
-        B.n_s = std::max<int64_t>(B.n_s, sect_size(T.K, nmax));
-        B.n_ax = std::max<int64_t>(B.n_ax, ax_size(T.Lt, T.Ls, T.K));
-        B.n_y = std::max<int64_t>(B.n_y, rows_size(T.Ls, T.K) * 4 * (int64_t)B.cls[i].ncols);
-        B.n_z = std::max<int64_t>(B.n_z, rows_size(T.Lt, T.K) * 4 * (int64_t)B.cls[i].ncols);
-        if (i > 0) {
-            const Task &P = B.gen[i - 1];
-            if (P.cth == T.cth && P.Ls == T.Ls && P.Lt == T.Lt && P.K == T.K) B.regen_d[i] = 0;
-            if (P.t == T.t && P.kind == T.kind && P.Ls == T.Ls && P.Lt == T.Lt && P.K == T.K) B.regen_ax[i] = 0;
+    const int nc = (int)B.gen.size();
+    B.chunks.clear();
+    B.descs.clear();
+    B.scat.clear();
+    B.scat_cols.clear();
+    if (nc == 0) return IFS_OK;
+    const Task &T0 = B.gen[0];
+    for (auto &T : B.gen)
+        if (T.Ls != T0.Ls || T.Lt != T0.Lt || T.K != T0.K) return IFS_E_ARG;  // one level pair per batch
+    // order classes by (radial kind, length, polar angle, azimuth): axial segments and
+    // rotation runs become consecutive
+    std::vector<int64_t> cb(nc + 1, 0);
+    for (int i = 0; i < nc; ++i) cb[i + 1] = cb[i] + B.ncols[i];
+    std::vector<int> ord(nc);
+    for (int i = 0; i < nc; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        const Task &x = B.gen[a], &y = B.gen[b];
+        if (x.kind != y.kind) return x.kind < y.kind;
+        if (x.t != y.t) return x.t < y.t;
+        if (x.cth != y.cth) return x.cth < y.cth;
+        return x.phi < y.phi;
+    });
+    std::vector<Task> gen(nc);
+    std::vector<int> ncols(nc);
+    std::vector<int64_t> src, dst;
+    B.pidx.clear();
+    for (int i = 0; i < nc; ++i) {
+        int o = ord[i];
+        gen[i] = B.gen[o];
+        ncols[i] = B.ncols[o];
+        for (int64_t j = cb[o]; j < cb[o + 1]; ++j) {
+            src.push_back(B.src[j]);
+            dst.push_back(B.dst[j]);
+            B.pidx.push_back(i);
         }
     }
+    B.gen.swap(gen);
+    B.ncols.swap(ncols);
+    B.src.swap(src);
+    B.dst.swap(dst);
+    // phase rows exp(i m phi), m = -PL..PL, evaluated in long double
+    B.PL = std::max(T0.Ls, T0.Lt);
+    const int PW = 2 * B.PL + 1;
+    B.phase.assign((size_t)nc * PW, make_double2(0.0, 0.0));
+    for (int i = 0; i < nc; ++i)
+        for (int m = -B.PL; m <= B.PL; ++m) {
+            long double a = (long double)m * (long double)B.gen[i].phi;
+            B.phase[(size_t)i * PW + m + B.PL] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+    B.plan_nrhs = 0;
+    return class_batch_plan(c, B, 1);
+}
+
+// Chunks, runs, segments and reduction lists for nrhs right-hand sides per call.
+int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
+    B.chunks.clear();
+    B.descs.clear();
+    B.scat.clear();
+    B.scat_cols.clear();
+    if (B.gen.empty()) return IFS_OK;
+    const Task &T0 = B.gen[0];
+    // chunk capacity from the workspace budget (Y/W and Z per column and RHS)
+    const int64_t ycol = 4 * std::max<int64_t>(rows_size(T0.Ls, T0.K), (int64_t)T0.Lt * (T0.Lt + 2));
+    const int64_t zcol = 4 * rows_size(T0.Lt, T0.K);
+    const int64_t per_col = 8 * (ycol + zcol) * nrhs;
+    const int64_t total = (int64_t)B.src.size();
+    int64_t cap = std::max<int64_t>(1, (int64_t)(c->ws_budget / (size_t)per_col));
+    cap = std::min<int64_t>(cap, total);
+    B.n_y = B.n_z = 0;
+    // walk the class columns, cutting chunks of at most cap columns
+    const int64_t nbox = 2LL * T0.Lt * (T0.Lt + 2);
+    int cls = 0;
+    int64_t done_in_cls = 0;
+    for (int64_t c0 = 0; c0 < total; c0 += cap) {
+        ClassBatch::ChunkPlan ch;
+        ch.col_begin = (int)c0;
+        ch.ncols = (int)std::min<int64_t>(cap, total - c0);
+        // pieces: class fragments inside the chunk
+        std::vector<std::pair<int, std::pair<int, int>>> frags;  // class, [begin, end) global columns
+        int64_t col = c0;
+        while (col < c0 + ch.ncols) {
+            int64_t left = B.ncols[cls] - done_in_cls;
+            int64_t take = std::min<int64_t>(left, c0 + ch.ncols - col);
+            frags.push_back({cls, {(int)col, (int)(col + take)}});
+            col += take;
+            done_in_cls += take;
+            if (done_in_cls == B.ncols[cls]) {
+                ++cls;
+                done_in_cls = 0;
+            }
+        }
+        auto piece = [&](int g, int b, int e) {
+            ClassBatch::Piece P;
+            P.gen = g;
+            GroupDesc &d = P.g;
+            d = GroupDesc{};
+            d.Ls = T0.Ls;
+            d.Lt = T0.Lt;
+            d.K = T0.K;
+            d.ncols = e - b;
+            d.col_begin = b;
+            d.ycol0 = b - (int)c0;
+            d.ncy = ch.ncols;
+            d.PL = B.PL;
+            d.nrhs = nrhs;
+            d.src_stride = B.src_stride;
+            d.dst_stride = B.dst_stride;
+            return P;
+        };
+        // rotation runs: consecutive fragments with one polar angle
+        for (size_t i = 0; i < frags.size();) {
+            size_t j = i + 1;
+            while (j < frags.size() && same_d(B.gen[frags[j].first], B.gen[frags[i].first])) ++j;
+            ch.runs.push_back(piece(frags[i].first, frags[i].second.first, frags[j - 1].second.second));
+            i = j;
+        }
+        // axial segments: consecutive fragments with one length and kind
+        for (size_t i = 0; i < frags.size();) {
+            size_t j = i + 1;
+            while (j < frags.size() && same_ax(B.gen[frags[j].first], B.gen[frags[i].first])) ++j;
+            ch.segs.push_back(piece(frags[i].first, frags[i].second.first, frags[j - 1].second.second));
+            i = j;
+        }
+        for (auto &P : ch.runs) {
+            P.di = (int)B.descs.size();
+            B.descs.push_back(P.g);
+        }
+        for (auto &P : ch.segs) {
+            P.di = (int)B.descs.size();
+            B.descs.push_back(P.g);
+        }
+        // per-target reduction lists (chunk-local columns, ascending per target)
+        std::vector<std::pair<int64_t, int32_t>> td;
+        for (int j = 0; j < ch.ncols; ++j) td.push_back({B.dst[c0 + j], j});
+        std::sort(td.begin(), td.end());
+        ch.scat_begin = (int)B.scat.size();
+        for (size_t i = 0; i < td.size();) {
+            size_t j = i;
+            ScatterSeg S;
+            S.dst = td[i].first;
+            S.beg = (int32_t)B.scat_cols.size();
+            while (j < td.size() && td[j].first == td[i].first) B.scat_cols.push_back(td[j++].second);
+            S.end = (int32_t)B.scat_cols.size();
+            B.scat.push_back(S);
+            i = j;
+        }
+        ch.scat_end = (int)B.scat.size();
+        B.chunks.push_back(ch);
+        B.n_y = std::max<int64_t>(B.n_y, ycol * ch.ncols * nrhs);
+        B.n_z = std::max<int64_t>(B.n_z, zcol * ch.ncols * nrhs);
+    }
+    (void)nbox;
+    const int L = std::max(T0.Ls, T0.Lt);
+    B.n_d = fold_size(L, T0.K);
+    B.n_s = sect_size(T0.K, T0.Lt + T0.Ls + 2);
+    B.n_ax = ax_size(T0.Lt, T0.Ls, T0.K);
     int rc;
     if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, B.n_d * 8))) ||
         (rc = c->ws_s.alloc(std::max<size_t>(c->ws_s.bytes, B.n_s * 16))) ||
@@ -264,63 +455,78 @@ int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
         (rc = c->ws_y.alloc(std::max<size_t>(c->ws_y.bytes, B.n_y * 8))) ||
         (rc = c->ws_z.alloc(std::max<size_t>(c->ws_z.bytes, B.n_z * 8))))
         return rc;
-    if ((rc = B.d_cls.alloc(sizeof(ClassDesc) * std::max<size_t>(1, nc))) ||
-        (rc = B.d_gen.alloc(sizeof(Task) * std::max<size_t>(1, nc))) ||
-        (rc = B.d_src.alloc(8 * std::max<size_t>(1, B.src.size()))) ||
-        (rc = B.d_dst.alloc(8 * std::max<size_t>(1, B.dst.size()))))
+    auto up = [](DevBuf &d, const void *h, size_t bytes) -> int {
+        int r = d.alloc(std::max<size_t>(bytes, 16));
+        if (r) return r;
+        if (bytes && hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return IFS_E_HIP;
+        return IFS_OK;
+    };
+    if ((rc = up(B.d_desc, B.descs.data(), sizeof(GroupDesc) * B.descs.size())) ||
+        (rc = up(B.d_gen, B.gen.data(), sizeof(Task) * B.gen.size())) ||
+        (rc = up(B.d_src, B.src.data(), 8 * B.src.size())) || (rc = up(B.d_pidx, B.pidx.data(), 4 * B.pidx.size())) ||
+        (rc = up(B.d_phase, B.phase.data(), sizeof(double2) * B.phase.size())) ||
+        (rc = up(B.d_scat, B.scat.data(), sizeof(ScatterSeg) * B.scat.size())) ||
+        (rc = up(B.d_scat_cols, B.scat_cols.data(), 4 * B.scat_cols.size())))
         return rc;
-    if (nc) {
-        HIP_OK(hipMemcpy(B.d_cls.p, B.cls.data(), sizeof(ClassDesc) * nc, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(B.d_gen.p, B.gen.data(), sizeof(Task) * nc, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(B.d_src.p, B.src.data(), 8 * B.src.size(), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(B.d_dst.p, B.dst.data(), 8 * B.dst.size(), hipMemcpyHostToDevice));
-    }
+    B.plan_nrhs = nrhs;
     return IFS_OK;
 }
 
-int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool) {
-    const ClassDesc *dc = B.d_cls.as<ClassDesc>();
+int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs) {
+    if (B.gen.empty() || nrhs <= 0) return IFS_OK;
+    if (B.plan_nrhs != nrhs) {
+        int rc = class_batch_plan(c, B, nrhs);
+        if (rc) return rc;
+    }
+    const GroupDesc *dd = B.d_desc.as<GroupDesc>();
     const Task *dg = B.d_gen.as<Task>();
+    const int64_t *dsrc = B.d_src.as<int64_t>();
+    const int32_t *dpidx = B.d_pidx.as<int32_t>();
+    const double2 *dph = B.d_phase.as<double2>();
     double *wd = c->ws_d.as<double>();
     double2 *wsc = c->ws_s.as<double2>();
     double2 *wax = c->ws_ax.as<double2>();
     double *wy = c->ws_y.as<double>();
     double *wz = c->ws_z.as<double>();
-    for (size_t i = 0; i < B.cls.size(); ++i) {
-        const Task &T = B.gen[i];
-        double ffwd = 0.0, fback = 0.0, fax = 0.0;
-        if (c->prof.on) {
-            for (int n = 1; n <= T.Ls; ++n) ffwd += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-            for (int n = 1; n <= T.Lt; ++n) fback += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-            for (int k = -T.K; k <= T.K; ++k) fax += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
-            ffwd *= B.cls[i].ncols;
-            fback *= B.cls[i].ncols;
-            fax *= B.cls[i].ncols;
+    const Task &T0 = B.gen[0];
+    double ffwd, fax, fback;
+    class_col_flops(T0, ffwd, fax, fback);
+    const int L = std::max(T0.Ls, T0.Lt);
+    const int64_t nbox = 2LL * T0.Lt * (T0.Lt + 2);
+    int cur_d = -1, cur_ax = -1;  // class whose table is in the workspace
+    auto need_d = [&](int g) {
+        if (cur_d >= 0 && same_d(B.gen[cur_d], B.gen[g])) return;
+        Scope s(c, "class_gen_wigner");
+        launch_gen_wigner_fold(dg + g, 1, (T0.K + 1) * (L + 1), wd, c->stream);
+        cur_d = g;
+    };
+    for (auto &ch : B.chunks) {
+        for (auto &P : ch.runs) {
+            need_d(P.gen);
+            Scope s(c, "class_rot_fwd", ffwd * P.g.ncols * nrhs);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, c->stream);
         }
-        if (B.regen_d[i]) {
-            Scope s(c, "class_gen_wigner");
-            int L = std::max(T.Ls, T.Lt);
-            launch_gen_wigner(dg + i, 1, (2 * T.K + 1) * (2 * L + 1), wd, c->stream);
+        for (auto &P : ch.segs) {
+            if (cur_ax < 0 || !same_ax(B.gen[cur_ax], B.gen[P.gen])) {
+                Scope s(c, "class_gen_axial");
+                launch_gen_sect(dg + P.gen, 1, T0.Lt + T0.Ls + 2, wsc, c->stream);
+                launch_gen_axial(dg + P.gen, 1, T0.K, T0.Lt + T0.Ls + 2, wsc, wax, c->stream);
+                cur_ax = P.gen;
+            }
+            Scope s(c, "class_axial", fax * P.g.ncols * nrhs);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, c->stream);
         }
-        if (B.regen_ax[i]) {
-            Scope s(c, "class_gen_axial");
-            launch_gen_sect(dg + i, 1, T.Lt + T.Ls + 2, wsc, c->stream);
-            launch_gen_axial(dg + i, 1, T.K, T.Lt + T.Ls + 2, wsc, wax, c->stream);
+        for (size_t r = ch.runs.size(); r-- > 0;) {
+            auto &P = ch.runs[r];
+            need_d(P.gen);
+            Scope s(c, "class_rot_back", fback * P.g.ncols * nrhs);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, c->stream);
         }
         {
-            Scope s(c, "class_rot_fwd", ffwd);
-            launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
-                         wy, wz, 0, c->stream);
-        }
-        {
-            Scope s(c, "class_axial", fax);
-            launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
-                         wy, wz, 1, c->stream);
-        }
-        {
-            Scope s(c, "class_rot_back", fback);
-            launch_class(B.cls[i], dc + i, B.d_src.as<int64_t>(), B.d_dst.as<int64_t>(), src_pool, dst_pool, wd, wax,
-                         wy, wz, 2, c->stream);
+            Scope s(c, "class_scatter");
+            launch_scatter(B.d_scat.as<ScatterSeg>() + ch.scat_begin, ch.scat_end - ch.scat_begin,
+                           B.d_scat_cols.as<int32_t>(), (const double2 *)wy, nbox, dst_pool, ch.ncols, nrhs,
+                           B.dst_stride, c->stream);
         }
     }
     if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
@@ -328,24 +534,20 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
 }
 
 void class_batch_cost(const ClassBatch &B, double &flops, double &bytes) {
-    for (size_t i = 0; i < B.cls.size(); ++i) {
-        const Task &T = B.gen[i];
-        double f = 0.0;
-        for (int n = 1; n <= T.Ls; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-        for (int n = 1; n <= T.Lt; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-        for (int k = -T.K; k <= T.K; ++k) f += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
-        flops += f * B.cls[i].ncols;
-        bytes += (4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5) * B.cls[i].ncols;
-    }
+    if (B.gen.empty()) return;
+    const Task &T = B.gen[0];
+    double ffwd, fax, fback;
+    class_col_flops(T, ffwd, fax, fback);
+    const double ncol = (double)B.src.size();
+    flops += (ffwd + fax + fback) * ncol;
+    bytes += (4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5) * ncol;
 }
 
 void batch_cost(const Batch &B, double &flops, double &bytes) {
     for (auto &T : B.tasks) {
-        double f = 0.0;
-        for (int n = 1; n <= T.Ls; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-        for (int n = 1; n <= T.Lt; ++n) f += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-        for (int k = -T.K; k <= T.K; ++k) f += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
-        flops += f;
+        double f1, f2, f3;
+        task_flops(T, f1, f2, f3);
+        flops += f1 + f2 + f3;
         bytes += 4.0 * T.Ls * (T.Ls + 2) * 16.0 * 0.5 + 4.0 * T.Lt * (T.Lt + 2) * 16.0 * 0.5;
     }
 }
@@ -467,7 +669,11 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     // workspace budget: a fraction of free memory
     size_t freeb = 0, totb = 0;
     (void)hipMemGetInfo(&freeb, &totb);
-    c->ws_budget = std::min<size_t>(freeb / 4, (size_t)16 << 30);
+    c->ws_budget = std::min<size_t>(freeb / 4, (size_t)48 << 30);
+    if (p->workspace_mb > 0) c->ws_budget = (size_t)p->workspace_mb << 20;
+    c->cache_budget = freeb / 5 * 2;
+    if (p->d_convention != IFS_D_PHYSICAL && p->d_convention != IFS_D_REFERENCE) return fail(IFS_E_ARG);
+    c->dsign = p->d_convention == IFS_D_REFERENCE ? 1.0 : -1.0;
 
     if (!p->use_fmm) {
         // direct all-pairs plan: for each target j, sources l != j in ascending order
@@ -478,13 +684,15 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
             if (!c->active[j]) continue;
             for (int l = 0; l < ns; ++l) {
                 if (l == j) continue;
-                double dv[3] = {c->pos[3 * l] - c->pos[3 * j], c->pos[3 * l + 1] - c->pos[3 * j + 1],
-                                c->pos[3 * l + 2] - c->pos[3 * j + 2]};
+                double dv[3] = {c->dsign * (c->pos[3 * l] - c->pos[3 * j]),
+                                c->dsign * (c->pos[3 * l + 1] - c->pos[3 * j + 1]),
+                                c->dsign * (c->pos[3 * l + 2] - c->pos[3 * j + 2])};
                 B.tasks.push_back(
                     make_task(dv, p->k, p->z_selector, c->Leff[l], c->Leff[j], c->off[l], c->off[j]));
             }
         }
-        if ((rc = batch_finalize(c, B, c->ws_budget, true)) != IFS_OK) return fail(rc);
+        if ((rc = batch_finalize(c, B, std::min<size_t>(c->ws_budget, (size_t)8 << 30), true)) != IFS_OK)
+            return fail(rc);
         batch_cost(B, c->flops_mv, c->bytes_mv);
     } else {
         if ((rc = fmm::build(c, &c->fmm)) != IFS_OK) return fail(rc);
@@ -520,29 +728,34 @@ void *ifs_get_stream(ifs_ctx *c) { return c ? (void *)c->stream : nullptr; }
 namespace ifs {
 // b = M x in the reference basis, or b_hat = S M S^-1 x_hat in the normalised
 // basis (normalised = 1), both in the flat layout.
-int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised) {
+int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb) {
+    if (nrhs < 1) return IFS_E_ARG;
+    if (ldb < 0) ldb = ld;
+    if (nrhs > 1 && (ld < c->nflat || ldb < c->nflat)) return IFS_E_ARG;
     const double *snm = normalised ? nullptr : c->d_snm.as<double>();
+    const int64_t nc = c->ncompact;
+    int rc;
+    if ((rc = c->d_xh.alloc(16ull * nc * nrhs)) || (rc = c->d_acc.alloc(16ull * nc * nrhs))) return rc;
     {
         Scope s(c, "pack_in");
         launch_pack_in(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_off.as<int64_t>(), snm, x, c->d_xh.as<double2>(),
-                       1, 0, 0, c->stream);
+                       nrhs, ld, nc, c->stream);
     }
     {
         Scope s(c, "zero");
-        launch_zero(c->d_acc.as<double2>(), c->ncompact, c->stream);
+        launch_zero(c->d_acc.as<double2>(), nc * nrhs, c->stream);
     }
-    int rc;
     if (c->fmm) {
-        rc = fmm::run(c, c->fmm, c->d_xh.as<double2>(), c->d_acc.as<double2>());
+        rc = fmm::run(c, c->fmm, c->d_xh.as<double2>(), c->d_acc.as<double2>(), nrhs, nc);
     } else {
-        rc = batch_run(c, c->direct, c->d_xh.as<double2>(), c->d_acc.as<double2>());
+        rc = batch_run(c, c->direct, c->d_xh.as<double2>(), c->d_acc.as<double2>(), nrhs, nc, nc);
     }
     if (rc != IFS_OK) return rc;
     {
         Scope s(c, "finalize");
         launch_finalize(c->ns, c->N, c->d_Leff.as<int32_t>(), c->d_active.as<int32_t>(), c->d_off.as<int64_t>(),
-                        snm, c->d_an.as<double2>(), c->d_bn.as<double2>(), x, c->d_acc.as<double2>(), b, 1, 0, 0,
-                        c->stream);
+                        snm, c->d_an.as<double2>(), c->d_bn.as<double2>(), x, c->d_acc.as<double2>(), b, nrhs, ld,
+                        ldb, nc, c->stream);
     }
     if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
     return IFS_OK;
@@ -558,7 +771,23 @@ extern "C" {
 
 int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
     if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
-    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0);
+    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n);
+}
+
+int ifs_matvec_batch_device(ifs_ctx *c, int32_t nrhs, const void *x_dev, void *b_dev, int64_t n, int64_t ld) {
+    if (!c || !x_dev || !b_dev || n != c->nflat || nrhs < 1 || ld < n) return IFS_E_ARG;
+    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, nrhs, ld);
+}
+
+int ifs_matvec_batch(ifs_ctx *c, int32_t nrhs, const double *x, double *b, int64_t n) {
+    if (!c || !x || !b || n != c->nflat || nrhs < 1) return IFS_E_ARG;
+    int rc;
+    if ((rc = c->d_x.alloc(16ull * n * nrhs)) || (rc = c->d_b.alloc(16ull * n * nrhs))) return rc;
+    HIP_OK(hipMemcpyAsync(c->d_x.p, x, 16ull * n * nrhs, hipMemcpyHostToDevice, c->stream));
+    if ((rc = matvec_impl(c, c->d_x.as<double2>(), c->d_b.as<double2>(), 0, nrhs, n))) return rc;
+    HIP_OK(hipMemcpyAsync(b, c->d_b.p, 16ull * n * nrhs, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return IFS_OK;
 }
 
 int ifs_matvec(ifs_ctx *c, const double *x, double *b, int64_t n) {

@@ -45,6 +45,7 @@ struct Chunk {
     int maxpairs = 0, maxK = 0, max_nmax = 0, max_Ls = 0;
     int64_t max_rows_y = 0, max_rows_z = 0;
     int rowback_blocks = 1;
+    double f_fwd = 0.0, f_ax = 0.0, f_back = 0.0;  // algorithmic flops of the chunk's products
 };
 
 // A translation batch: tasks from one source pool into one target pool.
@@ -57,6 +58,7 @@ struct Batch {
     bool cached = false;         // tables generated once and kept
     bool tables_ready = false;
     std::string name;
+    const char *pn[6] = {"gen_wigner", "gen_sect", "gen_axial", "rot_fwd", "axial_apply", "rot_back"};
 };
 
 struct Profiler {
@@ -129,7 +131,9 @@ struct ifs_ctx {
     ifs::DevBuf d_Leff, d_active, d_off, d_snm, d_an, d_bn, d_x, d_b;
     ifs::DevBuf d_xh, d_acc;             // compact normalised input / accumulator
     ifs::DevBuf ws_d, ws_s, ws_ax, ws_y, ws_z;
-    size_t ws_budget = 0;
+    double dsign = -1.0;            // d = dsign * (origin(source) - origin(target))
+    size_t ws_budget = 0;           // translation workspace (chunked batches)
+    size_t cache_budget = 0;        // tables kept across matvecs
     size_t ws_cached = 0;           // bytes held by cached batches
 
     ifs::Batch direct;              // sphere -> sphere (direct all pairs)
@@ -145,23 +149,52 @@ struct ifs_ctx {
 
 
 namespace ifs {
-// Translations grouped in classes that share one translation vector; run as
-// tiled products (ifs_gemm.hip) with one Wigner-d / coaxial table per class.
+// Translations grouped in classes that share one translation vector (one octree
+// offset), run as tiled products (ifs_gemm.hip).  At finalize the classes are
+// ordered by (length, polar angle) and cut into chunks that fit the workspace;
+// inside a chunk, consecutive classes with one polar angle form a rotation run
+// (one Wigner-d table) and consecutive classes with one length an axial segment
+// (one coaxial table).
 struct ClassBatch {
-    std::vector<ClassDesc> cls;
+    // input, filled by add_class
     std::vector<Task> gen;             // per class: table parameters (offsets 0)
-    std::vector<int64_t> src, dst;     // columns of all classes
-    std::vector<char> regen_d, regen_ax;
-    DevBuf d_cls, d_gen, d_src, d_dst;
+    std::vector<int> ncols;            // per class
+    std::vector<int64_t> src, dst;     // columns of all classes, class by class
+    // plan
+    struct Piece {
+        GroupDesc g;
+        int gen;                       // class whose Task generates the table
+        int di;                        // index of g in descs (device copy)
+    };
+    struct ChunkPlan {
+        int col_begin = 0, ncols = 0;
+        std::vector<Piece> runs, segs;
+        int scat_begin = 0, scat_end = 0;  // range in the scatter segment array
+    };
+    std::vector<ChunkPlan> chunks;
+    std::vector<GroupDesc> descs;
+    std::vector<int32_t> pidx;         // phase row per column
+    std::vector<double2> phase;        // per class: exp(i m phi), m = -PL..PL
+    std::vector<ScatterSeg> scat;
+    std::vector<int32_t> scat_cols;
+    int PL = 0;
+    int plan_nrhs = 0;                 // right-hand sides the plan was cut for
+    int64_t src_stride = 0, dst_stride = 0;  // pool stride between right-hand sides (complex)
+    DevBuf d_desc, d_gen, d_src, d_pidx, d_phase, d_scat, d_scat_cols;
     int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;  // workspace (doubles for d, y, z; complex for s, ax)
     std::string name;
 };
+// b_r = (I + T A) x_r for r < nrhs: x at stride ld, b at stride ldb (-1: ld); normalised
+// != 0 means x and b are in the normalised basis.
+int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb = -1);
 int class_batch_finalize(ifs_ctx *c, ClassBatch &B);
-int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool);
+int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs);
+int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs);
 void class_batch_cost(const ClassBatch &B, double &flops, double &bytes);
 
 int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache);
-int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool);
+int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, int nrhs = 1,
+              int64_t src_stride = 0, int64_t dst_stride = 0);
 void batch_cost(const Batch &B, double &flops, double &bytes);
 Task make_task(const double *dv, double k, int kind, int Ls, int Lt, int64_t src, int64_t dst);
 

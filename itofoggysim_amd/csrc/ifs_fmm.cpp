@@ -4,7 +4,8 @@
 // (libmlfmm/src/lib/lib_ml_fmm.f90:231-262 and the callbacks of
 // FoggySim/src/lib/mie_theory/lib_mie_ms_ml_fmm_interface.f90), with one
 // translation convention throughout: every translation uses
-// d = origin(source) - origin(target) in Xu's coefficients, regular waves (j_n)
+// d = dsign * (origin(source) - origin(target)) in Xu's coefficients (dsign = -1
+// by default, +1 for the reference's direct-path sign), regular waves (j_n)
 // for outgoing->outgoing (P2M, M2M) and regular->regular (L2L, L2P), and the
 // sphere-interaction kind (h1_n) for outgoing->regular (M2L, near field).
 //
@@ -59,26 +60,26 @@ static int box_L(double k, double edge, double d0, int cap) {
     return L;
 }
 
-// Appends one class (shared translation vector dv) with its columns.
+// Appends one class (shared translation vector dv) with its columns (source,
+// target offsets).  For an integer box offset o (o != nullptr, dv = o * edge) the
+// length and polar angle are taken from |o|^2 and o_z exactly, so that offsets
+// of one length (and one o_z) share their tables bit for bit.
 static void add_class(ClassBatch &B, const double *dv, double k, int kind, int Ls, int Lt,
-                      const std::vector<std::pair<int64_t, int64_t>> &cols) {
+                      const std::vector<std::pair<int64_t, int64_t>> &cols, const int *o = nullptr,
+                      double edge = 0.0) {
     if (cols.empty()) return;
     Task T = make_task(dv, k, kind, Ls, Lt, 0, 0);
     T.off_d = T.off_s = T.off_ax = T.off_y = T.off_z = 0;
-    ClassDesc C{};
-    C.off_d = 0;
-    C.off_ax = 0;
-    C.phi = T.phi;
-    C.Ls = Ls;
-    C.Lt = Lt;
-    C.K = T.K;
-    C.ncols = (int)cols.size();
-    C.col_begin = (int)B.src.size();
+    if (o) {
+        double r = std::sqrt((double)(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+        T.t = k * (edge * r);
+        T.cth = std::max(-1.0, std::min(1.0, (double)o[2] / r));
+    }
     for (auto &pr : cols) {
         B.src.push_back(pr.first);
         B.dst.push_back(pr.second);
     }
-    B.cls.push_back(C);
+    B.ncols.push_back((int)cols.size());
     B.gen.push_back(T);
 }
 
@@ -144,8 +145,11 @@ int build(ifs_ctx *c, Fmm **out) {
         delete f;
         return rc;
     }
-    auto dvec = [](const double *src, const double *dst, double *dv) {
-        for (int d = 0; d < 3; ++d) dv[d] = src[d] - dst[d];
+    // translation vector d = dsign * (origin(source) - origin(target))
+    const double sg = c->dsign;
+    const int isg = sg > 0 ? 1 : -1;
+    auto dvec = [sg](const double *src, const double *dst, double *dv) {
+        for (int d = 0; d < 3; ++d) dv[d] = sg * (src[d] - dst[d]);
     };
     const Level &LL = f->lv[lmax];
 
@@ -162,7 +166,7 @@ int build(ifs_ctx *c, Fmm **out) {
         f->counts[0] = (int64_t)f->p2m.tasks.size();
     }
     // M2M: child (level l+1) -> parent (level l), one class per child octant;
-    // d = centre(child) - centre(parent) = (b - 1/2) * edge(child)
+    // d = dsign * (centre(child) - centre(parent)) = dsign * (b - 1/2) * edge(child)
     f->m2m.resize(lmax + 1);
     for (int l = lmax - 1; l >= lmin; --l) {
         const Level &P = f->lv[l], &Ch = f->lv[l + 1];
@@ -172,14 +176,15 @@ int build(ifs_ctx *c, Fmm **out) {
             oct[Ch.boxes[q] & 7].push_back({Ch.off + (int64_t)q * Ch.per_box(), P.off + (int64_t)p * P.per_box()});
         }
         for (int b = 0; b < 8; ++b) {
-            double dv[3] = {(((b >> 2) & 1) - 0.5) * Ch.edge, (((b >> 1) & 1) - 0.5) * Ch.edge, ((b & 1) - 0.5) * Ch.edge};
+            double dv[3] = {sg * (((b >> 2) & 1) - 0.5) * Ch.edge, sg * (((b >> 1) & 1) - 0.5) * Ch.edge,
+                            sg * ((b & 1) - 0.5) * Ch.edge};
             std::sort(oct[b].begin(), oct[b].end(), [](auto &x, auto &y) { return x.second < y.second; });
             add_class(f->m2m[l], dv, k, 1, Ch.L, P.L, oct[b]);
             f->counts[1] += (int64_t)oct[b].size();
         }
     }
     // M2L: I4 of every box, one class per integer offset o = coords(src) - coords(tgt);
-    // d = o * edge.  Classes ordered by |o|^2, then o_z, so consecutive classes share tables.
+    // d = dsign * o * edge.  Classes ordered by |o|^2, then o_z, so consecutive classes share tables.
     f->m2l.resize(lmax + 1);
     for (int l = lmin; l <= lmax; ++l) {
         const Level &V = f->lv[l];
@@ -209,13 +214,14 @@ int build(ifs_ctx *c, Fmm **out) {
         for (auto &o : keys) {
             auto &cols = cls[o];
             std::sort(cols.begin(), cols.end(), [](auto &x, auto &y) { return x.second < y.second; });
-            double dv[3] = {o[0] * V.edge, o[1] * V.edge, o[2] * V.edge};
-            add_class(f->m2l[l], dv, k, c->prm.z_selector, V.L, V.L, cols);
+            int os[3] = {isg * o[0], isg * o[1], isg * o[2]};
+            double dv[3] = {os[0] * V.edge, os[1] * V.edge, os[2] * V.edge};
+            add_class(f->m2l[l], dv, k, c->prm.z_selector, V.L, V.L, cols, os, V.edge);
             f->counts[2] += (int64_t)cols.size();
         }
     }
     // L2L: parent (level l-1) -> child (level l), one class per child octant;
-    // d = centre(parent) - centre(child) = -(b - 1/2) * edge(child)
+    // d = dsign * (centre(parent) - centre(child)) = -dsign * (b - 1/2) * edge(child)
     f->l2l.resize(lmax + 1);
     for (int l = lmin + 1; l <= lmax; ++l) {
         const Level &P = f->lv[l - 1], &Ch = f->lv[l];
@@ -225,8 +231,8 @@ int build(ifs_ctx *c, Fmm **out) {
             oct[Ch.boxes[q] & 7].push_back({P.off + (int64_t)p * P.per_box(), Ch.off + (int64_t)q * Ch.per_box()});
         }
         for (int b = 0; b < 8; ++b) {
-            double dv[3] = {-(((b >> 2) & 1) - 0.5) * Ch.edge, -(((b >> 1) & 1) - 0.5) * Ch.edge,
-                            -((b & 1) - 0.5) * Ch.edge};
+            double dv[3] = {-sg * (((b >> 2) & 1) - 0.5) * Ch.edge, -sg * (((b >> 1) & 1) - 0.5) * Ch.edge,
+                            -sg * ((b & 1) - 0.5) * Ch.edge};
             add_class(f->l2l[l], dv, k, 1, P.L, Ch.L, oct[b]);
             f->counts[3] += (int64_t)oct[b].size();
         }
@@ -277,12 +283,13 @@ int build(ifs_ctx *c, Fmm **out) {
 
     auto fin = [&](Batch &B, const char *name) -> int {
         B.name = name;
-        int r = batch_finalize(c, B, c->ws_budget, true);
+        int r = batch_finalize(c, B, std::min<size_t>(c->ws_budget, (size_t)8 << 30), true);
         if (!r) batch_cost(B, c->flops_mv, c->bytes_mv);
         return r;
     };
     auto cfin = [&](ClassBatch &B, const char *name) -> int {
         B.name = name;
+        B.src_stride = B.dst_stride = f->nC;
         int r = class_batch_finalize(c, B);
         if (!r) class_batch_cost(B, c->flops_mv, c->bytes_mv);
         return r;
@@ -304,20 +311,23 @@ fail:
 
 void destroy(Fmm *f) { delete f; }
 
-int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc) {
-    double2 *C = f->C.as<double2>(), *D = f->D.as<double2>();
-    launch_zero(C, f->nC, c->stream);
-    launch_zero(D, f->nC, c->stream);
+int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t ld) {
     int rc;
-    if ((rc = batch_run(c, f->p2m, xh, C))) return rc;
+    // multipole / local pools hold nrhs copies at stride nC
+    if ((rc = f->C.alloc(16ull * f->nC * nrhs)) || (rc = f->D.alloc(16ull * f->nC * nrhs))) return rc;
+    double2 *C = f->C.as<double2>(), *D = f->D.as<double2>();
+    const int64_t st = f->nC;
+    launch_zero(C, f->nC * nrhs, c->stream);
+    launch_zero(D, f->nC * nrhs, c->stream);
+    if ((rc = batch_run(c, f->p2m, xh, C, nrhs, ld, st))) return rc;
     for (int l = f->l_max - 1; l >= f->l_min; --l)
-        if ((rc = class_batch_run(c, f->m2m[l], C, C))) return rc;
+        if ((rc = class_batch_run(c, f->m2m[l], C, C, nrhs))) return rc;
     for (int l = f->l_min; l <= f->l_max; ++l) {
-        if ((rc = class_batch_run(c, f->m2l[l], C, D))) return rc;
-        if (l > f->l_min && (rc = class_batch_run(c, f->l2l[l], D, D))) return rc;
+        if ((rc = class_batch_run(c, f->m2l[l], C, D, nrhs))) return rc;
+        if (l > f->l_min && (rc = class_batch_run(c, f->l2l[l], D, D, nrhs))) return rc;
     }
-    if ((rc = batch_run(c, f->l2p, D, acc))) return rc;
-    if ((rc = batch_run(c, f->p2p, xh, acc))) return rc;
+    if ((rc = batch_run(c, f->l2p, D, acc, nrhs, st, ld))) return rc;
+    if ((rc = batch_run(c, f->p2p, xh, acc, nrhs, ld, ld))) return rc;
     return IFS_OK;
 }
 

@@ -1,21 +1,24 @@
 // Class-batched translations as tiled FP64 matrix-core (MFMA) products (gfx950).
 //
-// A translation class is a set of translations that share one translation
-// vector (the same octree offset at one level, e.g. one M2L offset or one child
-// octant of M2M/L2L).  Its Wigner-d table and coaxial table are generated once
-// and every translation of the class becomes a column of three dense products:
+// Translations whose vectors share the polar angle (a rotation run: e.g. the
+// M2L offsets with one |o| and one o_z) share one Wigner-d table; translations
+// whose vectors share the length (an axial segment: one |o|) share one coaxial
+// table.  Every translation becomes a column of three dense products:
 //
 //   forward rotation, degree nu:  Y^nu (rows k)  = D^nu (k x m) * X^nu (m x cols)
 //   coaxial shift, order k:       Z_k^+ = P_k Y_k^+,  Z_k^- = Q_k Y_k^-            (complex)
 //   back rotation, degree n:      W^n  (rows m)  = D^n^T (m x k) * Z^n (k x cols)
 //
+// The rotations use the folded tables E, O of d^n (half the products' work).
+//
 // with x_+- = x_a +- x_b and P = A + B, Q = A - B (the [A B; B A] coaxial operator
 // decoupled).  Columns are the translations of the class, 4 reals each
 // (+.re, +.im, -.re, -.im).
-// The gather of the sources with the phase e^{i m phi} is fused into the loads of
-// the forward product, the phase e^{-i m phi} and the scatter-add into the targets
-// into the epilogue of the back product.  Targets of one class are distinct, so
-// the scatter needs no atomics.
+// The gather of the sources with the per-column phase e^{i m phi} is fused into
+// the loads of the forward product and the phase e^{-i m phi} into the epilogue
+// of the back product, which writes each column as a full expansion W.  A
+// separate pass adds the columns of each target in a fixed order (deterministic,
+// no atomics).
 #include <hip/hip_runtime.h>
 
 #include "ifs_internal.h"
@@ -34,92 +37,98 @@ __device__ __forceinline__ dv4 mfma(double a, double b, dv4 c) {
 
 __device__ __forceinline__ int fidx(int n, int m) { return n * n - 1 + m + n; }
 
-// Rotation products: block tile 64 rows x 128 real columns (32 translations),
-// 4 waves of 32 x 64, inner step 16 (four 16x16x4 steps).
-constexpr int RM = 64;
-constexpr int RN = 128;
-constexpr int RK = 16;
-constexpr int RPAD = 2;
+constexpr int RPAD = 2;  // LDS row padding (doubles)
 
 }  // namespace
 
-// Phase table e^{i m phi}, m = -L..L, into LDS (ph[m + L]).
-__device__ __forceinline__ void phase_table(double2 *ph, int L, double phi, double sign) {
-    for (int m = threadIdx.x; m <= 2 * L; m += blockDim.x) {
-        double sn, cs;
-        sincos(sign * (double)(m - L) * phi, &sn, &cs);
-        ph[m] = make_double2(cs, sn);
-    }
-}
+// Folded rotation products.  With E, O the folded Wigner tables (k, m >= 0):
+//   forward, degree nu:  S = E s,  T = O t,   s_m = u_m + (-1)^m u_{-m} (s_0 = u_0),
+//                        t_m = u_m - (-1)^m u_{-m},  u_m = e^{i m phi} x_m
+//                        y_k = (S_k + T_k)/2,  y_{-k} = (-1)^k (S_k - T_k)/2
+//   back, degree n:      F = E^T s,  G = O^T t,  s_k = z_k + (-1)^k z_{-k} (s_0 = z_0),
+//                        t_k = z_k - (-1)^k z_{-k}
+//                        w_m = (F_m + G_m)/2,  w_{-m} = (-1)^m (F_m - G_m)/2
+// which is half the work of the unfolded products.  Block: 32 rows of each
+// product x 128 real columns; waves 0,1 compute the E product, waves 2,3 the O
+// product (64 columns each), and the halves meet through LDS in the epilogue.
+constexpr int FM = 32;
+constexpr int FN = 128;
+constexpr int FK = 16;
 
-constexpr int PH_MAX = 2 * 1024 + 1;  // phase table entries (L <= 1024)
-
-// Forward rotation of one class: grid (col tiles, row tiles, degree-1).
-//   Y^nu[k][4j+p] = sum_m d^nu_{km} (e^{i m phi} x_j(nu,m))_p
-// The next inner step is prefetched into registers while the matrix cores work.
-__global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restrict__ cls,
+__global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restrict__ desc,
                                                        const int64_t *__restrict__ src_off,
+                                                       const int32_t *__restrict__ pidx,
+                                                       const double2 *__restrict__ phase,
                                                        const double2 *__restrict__ src_pool,
                                                        const double *__restrict__ wd, double *__restrict__ wy) {
-    const ClassDesc C = *cls;
+    const GroupDesc C = *desc;
     const int nu = blockIdx.z + 1;
     if (nu > C.Ls) return;
     const int K = C.K;
     const int kk = nu < K ? nu : K;
-    const int M = 2 * kk + 1;   // rows k = -kk..kk
-    const int KI = 2 * nu + 1;  // inner m = -nu..nu
-    const int NC = 4 * C.ncols;
-    const int row0 = blockIdx.y * RM;
-    const int col0 = blockIdx.x * RN;
+    const int M = kk + 1;   // rows k = 0..kk
+    const int KI = nu + 1;  // inner m = 0..nu
+    const int NTOT = C.ncols * C.nrhs;  // columns: translations x right-hand sides
+    const int NC = 4 * NTOT;
+    const int NCY = 4 * C.ncy * C.nrhs;
+    const int row0 = blockIdx.y * FM;
+    const int col0 = blockIdx.x * FN;
     if (row0 >= M) return;
-    __shared__ double As[RK][RM + RPAD];
-    __shared__ double Bs[RK][RN + RPAD];
-    __shared__ double2 ph[PH_MAX];
-    phase_table(ph, nu, C.phi, 1.0);
-    const double *A = wd + C.off_d + wig_off(nu, K);  // row-major (M x KI)
+    __shared__ double As[FK][2 * FM + RPAD];     // columns 0..31: E rows, 32..63: O rows
+    __shared__ double Bs[2][FK][FN + RPAD];      // s, t
+    const double *E = wd + C.off_d + fold_off(nu, K);
+    const double *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+    const int g = w >> 1;            // 0: E product, 1: O product
+    const int wc = (w & 1) * 64;
     dv4 acc[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
     const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
-    // gather: thread -> (translation t, inner q): idx = tid + 256 e, t = idx / 16, q = idx % 16
+    // gather: thread -> (translation t, inner q), idx = tid + 256 e, t = idx / 16, q = idx % 16
     const double2 *xs[2];
+    const double2 *ps[2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
         int t = (tid + 256 * e) >> 4;
         int j = (col0 >> 2) + t;
-        xs[e] = (j < C.ncols) ? src_pool + src_off[C.col_begin + j] + fidx(nu, 0) : nullptr;
+        bool ok = j < NTOT;
+        int tt = ok ? j % C.ncols : 0, rr = ok ? j / C.ncols : 0;  // translation, right-hand side
+        xs[e] = ok ? src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0) : nullptr;  // xs[m]
+        ps[e] = ok ? phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL : nullptr;
     }
-    __syncthreads();  // phase table ready
     double ra[4];
-    double2 rxa[2], rxb[2];
+    double2 rap[2], rbp[2], ram[2], rbm[2], rph[2];
     auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 64 rows x 16 inner
+            int idx = tid + 256 * e;  // 64 rows (E 32, O 32) x 16 inner
             int r = idx >> 4, q = idx & 15;
-            int gr = row0 + r, gq = k0 + q;
-            ra[e] = (gr < M && gq < KI) ? A[(int64_t)gr * KI + gq] : 0.0;
+            int gr = row0 + (r & 31), gq = k0 + q;
+            const double *src = (r < 32) ? E : O;
+            ra[e] = (gr < M && gq < KI) ? src[(int64_t)gr * KI + gq] : 0.0;
         }
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             int gq = k0 + ((tid + 256 * e) & 15);
             if (gq < KI && xs[e]) {
-                rxa[e] = xs[e][gq - nu];
-                rxb[e] = xs[e][gq - nu + nsrc];
+                rap[e] = xs[e][gq];
+                rbp[e] = xs[e][gq + nsrc];
+                ram[e] = xs[e][-gq];
+                rbm[e] = xs[e][-gq + nsrc];
+                rph[e] = ps[e][gq];
             } else {
-                rxa[e] = make_double2(0.0, 0.0);
-                rxb[e] = make_double2(0.0, 0.0);
+                rap[e] = rbp[e] = ram[e] = rbm[e] = make_double2(0.0, 0.0);
+                rph[e] = make_double2(0.0, 0.0);
             }
         }
     };
     load(0);
-    for (int k0 = 0; k0 < KI; k0 += RK) {
+    for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int idx = tid + 256 * e;
@@ -129,24 +138,39 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restri
         for (int e = 0; e < 2; ++e) {
             int idx = tid + 256 * e;
             int t = idx >> 4, q = idx & 15;
-            int gq = k0 + q;
-            double2 p = (gq < KI) ? ph[gq] : make_double2(1.0, 0.0);  // m + nu = gq
-            double2 xa = make_double2(rxa[e].x + rxb[e].x, rxa[e].y + rxb[e].y);  // x_+
-            double2 xb = make_double2(rxa[e].x - rxb[e].x, rxa[e].y - rxb[e].y);  // x_-
-            Bs[q][4 * t + 0] = p.x * xa.x - p.y * xa.y;
-            Bs[q][4 * t + 1] = p.x * xa.y + p.y * xa.x;
-            Bs[q][4 * t + 2] = p.x * xb.x - p.y * xb.y;
-            Bs[q][4 * t + 3] = p.x * xb.y + p.y * xb.x;
+            int m = k0 + q;
+            double2 p = rph[e];                          // e^{i m phi}
+            // u_m (+ and - parts) and u_{-m} (phase e^{-i m phi})
+            double xpr = rap[e].x + rbp[e].x, xpi = rap[e].y + rbp[e].y;
+            double xmr = rap[e].x - rbp[e].x, xmi = rap[e].y - rbp[e].y;
+            double ypr = ram[e].x + rbm[e].x, ypi = ram[e].y + rbm[e].y;
+            double ymr = ram[e].x - rbm[e].x, ymi = ram[e].y - rbm[e].y;
+            double u0r = p.x * xpr - p.y * xpi, u0i = p.x * xpi + p.y * xpr;   // u+_m
+            double u1r = p.x * xmr - p.y * xmi, u1i = p.x * xmi + p.y * xmr;   // u-_m
+            double v0r = p.x * ypr + p.y * ypi, v0i = p.x * ypi - p.y * ypr;   // u+_{-m}
+            double v1r = p.x * ymr + p.y * ymi, v1i = p.x * ymi - p.y * ymr;   // u-_{-m}
+            double sg = (m & 1) ? -1.0 : 1.0;
+            if (m == 0) {
+                sg = 0.0;  // s_0 = u_0, t_0 = 0 (O has a zero column 0)
+            }
+            Bs[0][q][4 * t + 0] = u0r + sg * v0r;
+            Bs[0][q][4 * t + 1] = u0i + sg * v0i;
+            Bs[0][q][4 * t + 2] = u1r + sg * v1r;
+            Bs[0][q][4 * t + 3] = u1i + sg * v1i;
+            Bs[1][q][4 * t + 0] = u0r - sg * v0r;
+            Bs[1][q][4 * t + 1] = u0i - sg * v0i;
+            Bs[1][q][4 * t + 2] = u1r - sg * v1r;
+            Bs[1][q][4 * t + 3] = u1i - sg * v1i;
         }
         __syncthreads();
-        if (k0 + RK < KI) load(k0 + RK);
+        if (k0 + FK < KI) load(k0 + FK);
 #pragma unroll
-        for (int kc = 0; kc < RK; kc += 4) {
+        for (int kc = 0; kc < FK; kc += 4) {
             double a[2], b[4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][wr + 16 * i + li];
+            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kc + lk][wc + 16 * j + li];
+            for (int j = 0; j < 4; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -154,22 +178,179 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const ClassDesc *__restri
         }
         __syncthreads();
     }
-    double *Y = wy + rows_off(nu, K) * (int64_t)NC;
+    // epilogue: the O-product waves hand T to the E-product waves through LDS
+    double *X = &Bs[0][0][0];  // 32 x (FN + RPAD) doubles
+    if (g == 1) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FN + RPAD) + wc + 16 * j + li] = acc[i][j][v];
+    }
+    __syncthreads();
+    if (g == 1) return;
+    double *Y = wy + rows_off(nu, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            int gc = col0 + wc + 16 * j + li;
+            int c = wc + 16 * j + li;
+            int gc = col0 + c;
             if (gc >= NC) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int gr = row0 + wr + 16 * i + 4 * v + lk;
-                if (gr < M) Y[(int64_t)gr * NC + gc] = acc[i][j][v];
+                int r = 16 * i + 4 * v + lk;
+                int k = row0 + r;
+                if (k >= M) continue;
+                double S = acc[i][j][v], T = X[r * (FN + RPAD) + c];
+                Y[(int64_t)(kk + k) * NCY + gc] = 0.5 * (S + T);
+                if (k > 0) Y[(int64_t)(kk - k) * NCY + gc] = ((k & 1) ? -0.5 : 0.5) * (S - T);
             }
         }
 }
 
-// Coaxial shift of one class: grid (translation tiles, row tiles, 2K+1 orders).
+__global__ __launch_bounds__(256) void k_class_rot_back(const GroupDesc *__restrict__ desc,
+                                                        const int32_t *__restrict__ pidx,
+                                                        const double2 *__restrict__ phase,
+                                                        const double *__restrict__ wd, const double *__restrict__ wz,
+                                                        double2 *__restrict__ W) {
+    const GroupDesc C = *desc;
+    const int n = blockIdx.z + 1;
+    if (n > C.Lt) return;
+    const int K = C.K;
+    const int kk = n < K ? n : K;
+    const int M = n + 1;    // rows m = 0..n
+    const int KI = kk + 1;  // inner k = 0..kk
+    const int NTOT = C.ncols * C.nrhs;
+    const int NC = 4 * NTOT;
+    const int NCY = 4 * C.ncy * C.nrhs;
+    const int row0 = blockIdx.y * FM;
+    const int col0 = blockIdx.x * FN;
+    if (row0 >= M) return;
+    __shared__ double As[FK][2 * FM + RPAD];
+    __shared__ double Bs[2][FK][FN + RPAD];
+    const double *E = wd + C.off_d + fold_off(n, K);  // E[k][m], KI x M
+    const double *O = E + (int64_t)KI * M;
+    const double *Z = wz + rows_off(n, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;  // row kk + k
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int g = w >> 1;
+    const int wc = (w & 1) * 64;
+    dv4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+    // raw loads only: the s/t combination happens at the LDS store, so that the
+    // prefetch of the next step stays in flight while the matrix cores work
+    double ra[4], rzp[8], rzm[8];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;  // 16 inner x 64 rows (E 32, O 32), coalesced along m
+            int q = idx >> 6, r = idx & 63;
+            int gm = row0 + (r & 31), gk = k0 + q;
+            const double *src = (r < 32) ? E : O;
+            ra[e] = (gm < M && gk < KI) ? src[(int64_t)gk * M + gm] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;  // 16 inner x 128 columns
+            int q = idx >> 7, c = idx & 127;
+            int gk = k0 + q, gc = col0 + c;
+            bool ok = gk < KI && gc < NC;
+            rzp[e] = ok ? Z[(int64_t)(kk + gk) * NCY + gc] : 0.0;
+            rzm[e] = (ok && gk > 0) ? Z[(int64_t)(kk - gk) * NCY + gc] : 0.0;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < KI; k0 += FK) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;
+            As[idx >> 6][idx & 63] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;
+            int q = idx >> 7;
+            int gk = k0 + q;
+            // s_k = z_k + (-1)^k z_{-k}, t_k = z_k - (-1)^k z_{-k}; s_0 = z_0, t_0 = 0 (rzm = 0 at k = 0)
+            double sg = (gk & 1) ? -1.0 : 1.0;
+            Bs[0][q][idx & 127] = rzp[e] + sg * rzm[e];
+            Bs[1][q][idx & 127] = rzp[e] - sg * rzm[e];
+        }
+        __syncthreads();
+        if (k0 + FK < KI) load(k0 + FK);
+#pragma unroll
+        for (int kc = 0; kc < FK; kc += 4) {
+            double a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    double *X = &Bs[0][0][0];
+    if (g == 1) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FN + RPAD) + wc + 16 * j + li] = acc[i][j][v];
+    }
+    __syncthreads();
+    if (g == 1) return;
+    // w_m and w_{-m}; the 4 parts of a translation sit in lanes li..li+3 (li % 4 == 0)
+    const int64_t ndst = (int64_t)C.Lt * (C.Lt + 2);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = 16 * i + 4 * v + lk;
+                int c = wc + 16 * j + li;
+                double F = acc[i][j][v], G = X[r * (FN + RPAD) + c];
+                double wp = 0.5 * (F + G), wm = 0.5 * (F - G);
+                double p1 = __shfl_down(wp, 1, 64), p2 = __shfl_down(wp, 2, 64), p3 = __shfl_down(wp, 3, 64);
+                double q1 = __shfl_down(wm, 1, 64), q2 = __shfl_down(wm, 2, 64), q3 = __shfl_down(wm, 3, 64);
+                int gc = col0 + c;
+                int m = row0 + r;
+                if ((li & 3) != 0 || gc >= NC || m >= M) continue;
+                int t = gc >> 2;
+                int tt = t % C.ncols, rr = t / C.ncols;  // translation, right-hand side
+                double2 ph = phase[(int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL + m];  // e^{i m phi}
+                double2 *dst = W + ((int64_t)rr * C.ncy + C.ycol0 + tt) * (2 * ndst);
+                {
+                    // w_m: parts (wp, p1) = z_+, (p2, p3) = z_-; a = (z_+ + z_-)/2, b = (z_+ - z_-)/2; phase e^{-i m phi}
+                    double ar = 0.5 * (wp + p2), ai = 0.5 * (p1 + p3);
+                    double br = 0.5 * (wp - p2), bi = 0.5 * (p1 - p3);
+                    int64_t o = fidx(n, m);
+                    dst[o] = make_double2(ph.x * ar + ph.y * ai, ph.x * ai - ph.y * ar);
+                    dst[o + ndst] = make_double2(ph.x * br + ph.y * bi, ph.x * bi - ph.y * br);
+                }
+                if (m > 0) {
+                    // w_{-m} = (-1)^m (F - G)/2; phase e^{+i m phi}
+                    double sg = (m & 1) ? -1.0 : 1.0;
+                    double ar = 0.5 * sg * (wm + q2), ai = 0.5 * sg * (q1 + q3);
+                    double br = 0.5 * sg * (wm - q2), bi = 0.5 * sg * (q1 - q3);
+                    int64_t o = fidx(n, -m);
+                    dst[o] = make_double2(ph.x * ar - ph.y * ai, ph.x * ai + ph.y * ar);
+                    dst[o + ndst] = make_double2(ph.x * br - ph.y * bi, ph.x * bi + ph.y * br);
+                }
+            }
+}
+
+// Coaxial shift of one segment: grid (translation tiles, row tiles, 2K+1 orders).
 //   Z^+ = P Y^+,  Z^- = Q Y^-  (complex), computed as real products of
 //   [[Pr, -Pi], [Pi, Pr]] with [Yr; Yi].  One 16x16x4 tile = 8 rows n x 2 parts
 //   (tile row r: n = r % 8, part = r / 8), inner = 2 nu x 2 parts, 16 translations.
@@ -178,10 +359,10 @@ constexpr int AM = 64;
 constexpr int AN = 64;
 constexpr int AK = 8;
 
-__global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict__ cls,
+__global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict__ desc,
                                                      const double2 *__restrict__ wax, const double *__restrict__ wy,
                                                      double *__restrict__ wz) {
-    const ClassDesc C = *cls;
+    const GroupDesc C = *desc;
     const int K = C.K;
     const int k = (int)blockIdx.z - K;
     const int ak = k < 0 ? -k : k;
@@ -192,7 +373,9 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
     const int row0 = blockIdx.y * AM;
     const int col0 = blockIdx.x * AN;
     if (row0 >= M) return;
-    const int NC = 4 * C.ncols;
+    const int NC = 4 * C.ncy * C.nrhs;                     // row stride of Y and Z
+    const int64_t cb = 4 * (int64_t)C.ycol0 * C.nrhs;      // first real column of the segment
+    const int NTOT = C.ncols * C.nrhs;                     // columns of the segment
     const double2 *Pb = wax + C.off_ax + ax_off(k, Lt, Ls, K);
     const double2 *Qb = Pb + ax_block(k, Lt, Ls);
     __shared__ double2 Ps[AK][AM];
@@ -232,9 +415,9 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
             int nu = q + k0 + c;
             int j = col0 + (rr >> 2);
             double v = 0.0;
-            if (nu <= Ls && j < C.ncols) {
+            if (nu <= Ls && j < NTOT) {
                 int kq = nu < K ? nu : K;
-                v = wy[(rows_off(nu, K) + (k + kq)) * (int64_t)NC + 4 * (int64_t)col0 + rr];
+                v = wy[(rows_off(nu, K) + (k + kq)) * (int64_t)NC + cb + 4 * (int64_t)col0 + rr];
             }
             py[e] = v;
         }
@@ -289,11 +472,11 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
             int n = q + row0 + w * 16 + 8 * i + 4 * h + lk;
             if (n > Lt) continue;
             int kq = n < K ? n : K;
-            double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC;
+            double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC + cb;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int t = col0 + 16 * j + li;
-                if (t >= C.ncols) continue;
+                if (t >= NTOT) continue;
                 row[4 * t + 0] = accP[i][j][h];
                 row[4 * t + 1] = accP[i][j][h + 2];
                 row[4 * t + 2] = accQ[i][j][h];
@@ -302,135 +485,53 @@ __global__ __launch_bounds__(256) void k_class_axial(const ClassDesc *__restrict
         }
 }
 
-// Back rotation of one class with fused phase and scatter-add:
-// grid (col tiles, row tiles, degree-1); rows m = -n..n, inner k.
-__global__ __launch_bounds__(256) void k_class_rot_back(const ClassDesc *__restrict__ cls,
-                                                        const int64_t *__restrict__ dst_off,
-                                                        const double *__restrict__ wd, const double *__restrict__ wz,
-                                                        double2 *__restrict__ dst_pool) {
-    const ClassDesc C = *cls;
-    const int n = blockIdx.z + 1;
-    if (n > C.Lt) return;
-    const int K = C.K;
-    const int kk = n < K ? n : K;
-    const int M = 2 * n + 1;    // rows m
-    const int KI = 2 * kk + 1;  // inner k
-    const int NC = 4 * C.ncols;
-    const int row0 = blockIdx.y * RM;
-    const int col0 = blockIdx.x * RN;
-    if (row0 >= M) return;
-    __shared__ double As[RK][RM + RPAD];
-    __shared__ double Bs[RK][RN + RPAD];
-    __shared__ double2 ph[RM];
-    for (int r = threadIdx.x; r < RM; r += blockDim.x) {
-        double sn, cs;
-        sincos(-(double)(row0 + r - n) * C.phi, &sn, &cs);
-        ph[r] = make_double2(cs, sn);
+// Deterministic per-target sum of back-rotated columns: grid (targets, element blocks).
+// grid (targets, element blocks, right-hand sides).
+__global__ __launch_bounds__(256) void k_scatter(const ScatterSeg *__restrict__ segs, const int32_t *__restrict__ cols,
+                                                 const double2 *__restrict__ W, int64_t nbox,
+                                                 double2 *__restrict__ dst_pool, int ncy, int64_t dst_stride) {
+    const ScatterSeg S = segs[blockIdx.x];
+    const int r = blockIdx.z;
+    double2 *dst = dst_pool + r * dst_stride + S.dst;
+    const double2 *Wr = W + (int64_t)r * ncy * nbox;
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < nbox; e += (int64_t)blockDim.x * gridDim.y) {
+        double2 acc = dst[e];
+        for (int i = S.beg; i < S.end; ++i) {
+            double2 v = Wr[(int64_t)cols[i] * nbox + e];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        dst[e] = acc;
     }
-    const double *D = wd + C.off_d + wig_off(n, K);  // (KI x M) row-major: D[k][m]
-    const double *Z = wz + rows_off(n, K) * (int64_t)NC;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, w = tid >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
-    dv4 acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-    double ra[4], rb[8];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 16 inner x 64 rows, coalesced along m
-            int qq = idx >> 6, r = idx & 63;
-            int gr = row0 + r, gq = k0 + qq;
-            ra[e] = (gr < M && gq < KI) ? D[(int64_t)gq * M + gr] : 0.0;
-        }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            int idx = tid + 256 * e;  // 16 inner x 128 columns
-            int qq = idx >> 7, cc = idx & 127;
-            int gq = k0 + qq, gc = col0 + cc;
-            rb[e] = (gq < KI && gc < NC) ? Z[(int64_t)gq * NC + gc] : 0.0;
-        }
-    };
-    load(0);
-    for (int k0 = 0; k0 < KI; k0 += RK) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;
-            As[idx >> 6][idx & 63] = ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            int idx = tid + 256 * e;
-            Bs[idx >> 7][idx & 127] = rb[e];
-        }
-        __syncthreads();
-        if (k0 + RK < KI) load(k0 + RK);
-#pragma unroll
-        for (int kc = 0; kc < RK; kc += 4) {
-            double a[2], b[4];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][wr + 16 * i + li];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kc + lk][wc + 16 * j + li];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-        }
-        __syncthreads();
-    }
-    // epilogue: the 4 parts of a translation sit in lanes li..li+3 (li % 4 == 0)
-    const int64_t ndst = (int64_t)C.Lt * (C.Lt + 2);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double c0 = acc[i][j][v];
-                double c1 = __shfl_down(c0, 1, 64);
-                double c2 = __shfl_down(c0, 2, 64);
-                double c3 = __shfl_down(c0, 3, 64);
-                int gc = col0 + wc + 16 * j + li;
-                int lr = wr + 16 * i + 4 * v + lk;
-                int gr = row0 + lr;
-                if ((li & 3) != 0 || gc >= NC || gr >= M) continue;
-                int t = gc >> 2;
-                int m = gr - n;
-                double2 p = ph[lr];
-                double2 *dst = dst_pool + dst_off[C.col_begin + t];
-                int64_t o = fidx(n, m);
-                // (c0, c1) = back-rotated z_+, (c2, c3) = z_-: a = (z_+ + z_-)/2, b = (z_+ - z_-)/2
-                double ar = 0.5 * (c0 + c2), ai = 0.5 * (c1 + c3);
-                double br = 0.5 * (c0 - c2), bi = 0.5 * (c1 - c3);
-                double2 da = dst[o], db = dst[o + ndst];
-                dst[o] = make_double2(da.x + p.x * ar - p.y * ai, da.y + p.x * ai + p.y * ar);
-                dst[o + ndst] = make_double2(db.x + p.x * br - p.y * bi, db.y + p.x * bi + p.y * br);
-            }
 }
 
-void launch_class(const ClassDesc &h, const ClassDesc *d_cls, const int64_t *src_off, const int64_t *dst_off,
-                  const double2 *src_pool, double2 *dst_pool, const double *wd, const double2 *wax, double *wy,
+void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
+                  const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
                   double *wz, int stage, hipStream_t s) {
-    const int NC = 4 * h.ncols;
+    const int NC = 4 * h.ncols * h.nrhs;
     const int K = h.K;
+    if (h.ncols <= 0 || h.nrhs <= 0) return;
     if (stage == 0) {
-        int mt = (2 * (h.Ls < K ? h.Ls : K) + 1 + RM - 1) / RM;
-        dim3 grid((NC + RN - 1) / RN, mt, h.Ls);
-        hipLaunchKernelGGL(k_class_rot_fwd, grid, dim3(256), 0, s, d_cls, src_off, src_pool, wd, wy);
+        int mt = ((h.Ls < K ? h.Ls : K) + 1 + FM - 1) / FM;
+        dim3 grid((NC + FN - 1) / FN, mt, h.Ls);
+        hipLaunchKernelGGL(k_class_rot_fwd, grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
-        dim3 grid((h.ncols + AN - 1) / AN, mt, 2 * K + 1);
-        hipLaunchKernelGGL(k_class_axial, grid, dim3(256), 0, s, d_cls, wax, wy, wz);
+        dim3 grid((h.ncols * h.nrhs + AN - 1) / AN, mt, 2 * K + 1);
+        hipLaunchKernelGGL(k_class_axial, grid, dim3(256), 0, s, d_desc, wax, wy, wz);
     } else {
-        int mt = (2 * h.Lt + 1 + RM - 1) / RM;
-        dim3 grid((NC + RN - 1) / RN, mt, h.Lt);
-        hipLaunchKernelGGL(k_class_rot_back, grid, dim3(256), 0, s, d_cls, dst_off, wd, wz, dst_pool);
+        int mt = (h.Lt + 1 + FM - 1) / FM;
+        dim3 grid((NC + FN - 1) / FN, mt, h.Lt);
+        hipLaunchKernelGGL(k_class_rot_back, grid, dim3(256), 0, s, d_desc, pidx, phase, wd, wz, (double2 *)wy);
     }
+}
+
+void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,
+                    double2 *dst_pool, int ncy, int nrhs, int64_t dst_stride, hipStream_t s) {
+    if (nsegs <= 0 || nrhs <= 0) return;
+    int yb = (int)std::min<int64_t>((nbox + 255) / 256, 64);
+    hipLaunchKernelGGL(k_scatter, dim3(nsegs, yb, nrhs), dim3(256), 0, s, segs, cols, W, nbox, dst_pool, ncy,
+                       dst_stride);
 }
 
 }  // namespace ifs

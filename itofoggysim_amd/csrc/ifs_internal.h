@@ -34,16 +34,36 @@ struct Task {
     int32_t kind;    // 1: regular (j_n), 3: outgoing -> regular (h1_n)
 };
 
-// A translation class: columns [col_begin, col_begin + ncols) of the class
-// source/target offset arrays share one translation vector and truncations.
-struct ClassDesc {
-    int64_t off_d;    // Wigner-d table (doubles) in the d workspace
-    int64_t off_ax;   // coaxial table (complex) in the coaxial workspace
-    double phi;       // azimuth of the translation vector
+// Work descriptor of the class-batched products (ifs_gemm.hip).  Translations
+// are columns; a rotation run is a column range whose translation vectors share
+// the polar angle (one Wigner-d table), an axial segment a column range whose
+// vectors share the length and radial kind (one coaxial table).  Columns live in
+// a chunk workspace: Y and Z are row-major with 4*ncy doubles per row, W (the
+// back-rotated columns) is column-major, 2*Lt(Lt+2) complex per column.
+//
+// With several right-hand sides every translation is one column per RHS: a run
+// or segment covers ncols * nrhs workspace columns starting at ycol0 * nrhs (RHS r,
+// translation t at column ycol0 * nrhs + r * ncols + t), and W holds RHS r of the
+// chunk's translation c at column r * ncy + c.  Source and target pools hold the
+// RHS at strides src_stride / dst_stride (complex).
+struct GroupDesc {
+    int64_t off_d;     // Wigner-d table (doubles) in the d workspace
+    int64_t off_ax;    // coaxial table (complex) in the coaxial workspace
     int32_t Ls, Lt, K;
-    int32_t ncols;
-    int32_t col_begin;
+    int32_t ncols;     // translations of this run / segment
+    int32_t col_begin; // first column in the batch column arrays (source offset, phase row)
+    int32_t ycol0;     // first translation in the chunk workspace
+    int32_t ncy;       // translations of the chunk
+    int32_t PL;        // phase rows hold exp(i m phi), m = -PL..PL
+    int32_t nrhs;      // right-hand sides
     int32_t pad;
+    int64_t src_stride, dst_stride;
+};
+
+// Deterministic reduction of back-rotated columns into one target expansion.
+struct ScatterSeg {
+    int64_t dst;       // complex offset of the target expansion (a-part)
+    int32_t beg, end;  // range in the chunk's sorted column list
 };
 
 // A run of consecutive tasks that share one target expansion.
@@ -63,6 +83,20 @@ __host__ __device__ inline int64_t wig_size(int Lmax, int K) {
     return s;
 }
 __host__ __device__ inline int64_t wig_off(int n, int K) { return wig_size(n - 1, K); }
+
+// Folded Wigner tables: per degree n (1..L), with kk = min(n,K), the matrices
+//   E[k][m] = d^n_{km} + (-1)^m d^n_{k,-m},  O[k][m] = d^n_{km} - (-1)^m d^n_{k,-m}
+// for k = 0..kk, m = 0..n (so E[k][0] = 2 d^n_{k0}, O[k][0] = O[0][m] = 0), row-major,
+// E then O.  d_{-k,-m} = (-1)^{k-m} d_{km} makes them carry all of d^n.
+__host__ __device__ inline int64_t fold_size(int L, int K) {
+    // sum_{n=1}^{L} 2 (min(n,K)+1)(n+1)
+    int nk = L < K ? L : K;
+    int64_t a = nk + 1;
+    int64_t s = 2 * (a * (a + 1) * (2 * a + 1) / 6 - 1);  // n = 1..nk: 2 (n+1)^2
+    if (L > K) s += (int64_t)(K + 1) * ((int64_t)(L + 1) * (L + 2) - (int64_t)(K + 1) * (K + 2));
+    return s;
+}
+__host__ __device__ inline int64_t fold_off(int n, int K) { return fold_size(n - 1, K); }
 
 __host__ __device__ inline int64_t rows_size(int L, int K) {
     // sum_{n=1}^{L} (2 min(n,K)+1)
@@ -101,6 +135,7 @@ __host__ __device__ inline int64_t sect_size(int K, int nmax) { return (int64_t)
 
 // Kernel launchers (ifs_kernels.hip).  All enqueue on `s`.
 void launch_gen_wigner(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s);
+void launch_gen_wigner_fold(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s);
 void launch_gen_sect(const Task *tasks, int ntasks, int max_nmax, double2 *wsect, hipStream_t s);
 void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect, double2 *wax,
                       hipStream_t s);
@@ -111,16 +146,23 @@ void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const d
 void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblocks, const double *wd,
                      const double2 *wz, double2 *dst_pool, hipStream_t s);
 
-void launch_class(const ClassDesc &h, const ClassDesc *d_cls, const int64_t *src_off, const int64_t *dst_off,
-                  const double2 *src_pool, double2 *dst_pool, const double *wd, const double2 *wax, double *wy,
+// stage 0: forward rotation of a run (source gather with exp(i m phi)) -> Y
+// stage 1: coaxial shift of a segment Y -> Z
+// stage 2: back rotation of a run (exp(-i m phi)) Z -> W (aliases Y)
+void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
+                  const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
                   double *wz, int stage, hipStream_t s);
+// dst_pool[r dst_stride + seg.dst + e] += sum_{i in [beg,end)} W[r ncy + cols[i]][e],
+// e < nbox = 2 Lt(Lt+2), r < nrhs
+void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,
+                    double2 *dst_pool, int ncy, int nrhs, int64_t dst_stride, hipStream_t s);
 
 void launch_mie(int ns, const double *xsize, const double2 *mrel, int N, double2 *an, double2 *bn, hipStream_t s);
 void launch_pack_in(int ns, int N, const int32_t *Leff, const int64_t *off, const double *snm, const double2 *x,
                     double2 *xh, int nrhs, int64_t ldx, int64_t ldxh, hipStream_t s);
 void launch_finalize(int ns, int N, const int32_t *Leff, const int32_t *active, const int64_t *off,
                      const double *snm, const double2 *an, const double2 *bn, const double2 *x, const double2 *acc,
-                     double2 *b, int nrhs, int64_t ldx, int64_t ldacc, hipStream_t s);
+                     double2 *b, int nrhs, int64_t ldx, int64_t ldb, int64_t ldacc, hipStream_t s);
 void launch_zero(double2 *p, int64_t n, hipStream_t s);
 void launch_scale_flat(int ns, int N, const double *snm, const double2 *x, double2 *y, int inverse, hipStream_t s);
 

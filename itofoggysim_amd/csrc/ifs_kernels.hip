@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include "ifs_internal.h"
+#include "ifs_wigner.h"
 
 namespace ifs {
 
@@ -62,41 +63,6 @@ __device__ __forceinline__ double coef_b(int n, int m) {
 // ------------------------------------------------------------------------
 // Wigner d tables
 // ------------------------------------------------------------------------
-// d^{n0}_{km}(beta) at n0 = max(|k|,|m|) from the closed forms
-//   d^j_{j,m}  = sqrt((2j)!/((j+m)!(j-m)!)) c^{j+m} (-s)^{j-m}
-//   d^j_{-j,m} = sqrt((2j)!/((j+m)!(j-m)!)) c^{j-m} s^{j+m}
-//   d^j_{k,m}  = (-1)^{k-m} d^j_{m,k}
-// with c = cos(beta/2), s = sin(beta/2), evaluated in logs.
-__device__ double wigner_seed(int k, int m, double c, double s) {
-    double sign = 1.0;
-    if ((k < 0 ? -k : k) < (m < 0 ? -m : m)) {
-        int t = k;
-        k = m;
-        m = t;
-        if ((k - m) & 1) sign = -sign;
-    }
-    int j = k < 0 ? -k : k;
-    int ec, es;
-    if (k >= 0) {
-        ec = j + m;
-        es = j - m;
-        if (es & 1) sign = -sign;
-    } else {
-        ec = j - m;
-        es = j + m;
-    }
-    double lv = 0.5 * (lgamma(2.0 * j + 1.0) - lgamma((double)(j + m) + 1.0) - lgamma((double)(j - m) + 1.0));
-    if (ec > 0) {
-        if (c <= 0.0) return 0.0;
-        lv += ec * log(c);
-    }
-    if (es > 0) {
-        if (s <= 0.0) return 0.0;
-        lv += es * log(s);
-    }
-    return sign * exp(lv);
-}
-
 // One workgroup strip per task (grid.y), threads over (k,m) pairs.
 __global__ void k_gen_wigner(const Task *__restrict__ tasks, double *__restrict__ wd) {
     const Task T = tasks[blockIdx.y];
@@ -150,6 +116,68 @@ void launch_gen_wigner(const Task *tasks, int ntasks, int maxpairs, double *wd, 
     if (ntasks <= 0) return;
     dim3 grid((maxpairs + 255) / 256, ntasks);
     hipLaunchKernelGGL(k_gen_wigner, grid, dim3(256), 0, s, tasks, wd);
+}
+
+// Folded tables (layout at fold_off): threads over (k, m), k = 0..K, m = 0..L; each
+// runs the recurrences of d_{k,m} and d_{k,-m} side by side from n0 = max(k, m).
+__global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__restrict__ wd) {
+    const Task T = tasks[blockIdx.y];
+    const int K = T.K;
+    const int L = T.Ls > T.Lt ? T.Ls : T.Lt;
+    const int width = L + 1;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (K + 1) * width) return;
+    const int k = p / width;
+    const int m = p % width;
+    const int n0 = k > m ? k : m;
+    const double cb = T.cth;
+    const double ch = sqrt(fmax(0.0, 0.5 * (1.0 + cb)));
+    const double sh = sqrt(fmax(0.0, 0.5 * (1.0 - cb)));
+    const double sgm = (m & 1) ? -1.0 : 1.0;
+    double *base = wd + T.off_d;
+    auto put = [&](int n, double dp, double dm) {
+        int kk = n < K ? n : K;
+        int64_t o = fold_off(n, K) + (int64_t)k * (n + 1) + m;
+        double e, od;
+        if (m == 0) {
+            e = 2.0 * dp;
+            od = 0.0;
+        } else {
+            e = dp + sgm * dm;
+            od = (k == 0) ? 0.0 : dp - sgm * dm;
+        }
+        base[o] = e;
+        base[o + (int64_t)(kk + 1) * (n + 1)] = od;
+    };
+    double p1, p2, q1, q2;  // chains d_{k,m} and d_{k,-m}: values at n-1 and n-2
+    int nstart;
+    if (n0 == 0) {
+        p2 = q2 = 1.0;  // d^0_{00}
+        p1 = q1 = cb;   // d^1_{00}
+        if (L >= 1) put(1, p1, q1);
+        nstart = 2;
+    } else {
+        p2 = q2 = 0.0;
+        p1 = wigner_seed(k, m, ch, sh);
+        q1 = (m == 0) ? p1 : wigner_seed(k, -m, ch, sh);
+        put(n0, p1, q1);
+        nstart = n0 + 1;
+    }
+    for (int n = nstart; n <= L; ++n) {
+        double pn = wigner_step(n, k, m, cb, p1, p2);
+        double qn = (m == 0) ? pn : wigner_step(n, k, -m, cb, q1, q2);
+        put(n, pn, qn);
+        p2 = p1;
+        p1 = pn;
+        q2 = q1;
+        q1 = qn;
+    }
+}
+
+void launch_gen_wigner_fold(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s) {
+    if (ntasks <= 0) return;
+    dim3 grid((maxpairs + 255) / 256, ntasks);
+    hipLaunchKernelGGL(k_gen_wigner_fold, grid, dim3(256), 0, s, tasks, wd);
 }
 
 // ------------------------------------------------------------------------
@@ -572,14 +600,14 @@ __global__ void k_finalize(int N, const int32_t *__restrict__ Leff, const int32_
                            const int64_t *__restrict__ off, const double *__restrict__ snm,
                            const double2 *__restrict__ an, const double2 *__restrict__ bn,
                            const double2 *__restrict__ x, const double2 *__restrict__ acc, double2 *__restrict__ b,
-                           int64_t ldx, int64_t ldacc) {
+                           int64_t ldx, int64_t ldb, int64_t ldacc) {
     const int j = blockIdx.x;
     const int r = blockIdx.z;
     const int L = Leff[j];
     const int blk = N * (N + 2);
     const int cnt = L * (L + 2);
     const double2 *xs = x + r * ldx + (int64_t)j * 2 * blk;
-    double2 *bs = b + r * ldx + (int64_t)j * 2 * blk;
+    double2 *bs = b + r * ldb + (int64_t)j * 2 * blk;
     const double2 *as = acc + r * ldacc + off[j];
     const int act = active[j];
     for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < blk; e += blockDim.x * gridDim.y) {
@@ -599,11 +627,11 @@ __global__ void k_finalize(int N, const int32_t *__restrict__ Leff, const int32_
 
 void launch_finalize(int ns, int N, const int32_t *Leff, const int32_t *active, const int64_t *off,
                      const double *snm, const double2 *an, const double2 *bn, const double2 *x, const double2 *acc,
-                     double2 *b, int nrhs, int64_t ldx, int64_t ldacc, hipStream_t s) {
+                     double2 *b, int nrhs, int64_t ldx, int64_t ldb, int64_t ldacc, hipStream_t s) {
     if (ns <= 0) return;
     int yb = (N * (N + 2) + 255) / 256;
     hipLaunchKernelGGL(k_finalize, dim3(ns, yb, nrhs), dim3(256), 0, s, N, Leff, active, off, snm, an, bn, x, acc,
-                       b, ldx, ldacc);
+                       b, ldx, ldb, ldacc);
 }
 
 // y = s * x (or x / s) per flat block entry, all n <= N (reference layout)

@@ -1,0 +1,257 @@
+// Serving loop: continuous batching of scattering requests on one GPU.
+//
+// Requests (a cluster, its parameters and 1..n incidence directions) are queued
+// as they arrive.  Worker slots take the oldest queued request whose estimated HBM
+// footprint fits what the admitted requests leave of the budget, so requests that
+// arrive while others run are admitted as soon as a slot and memory free up.
+// Every admitted request owns a context with its own HIP stream: its tables and
+// Krylov state stay device-resident for the whole request, and the work of
+// concurrent requests overlaps on the device.  A request's result does not depend
+// on what runs beside it (no shared device state, deterministic kernels).
+//
+// Per request: right-hand sides T p for every direction, lock-step GMRES over the
+// directions (one batched matvec per Arnoldi step), then the cross sections.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/ifs.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Req {
+    int64_t id = 0;
+    std::vector<double> centres, radii, mrel, alpha, beta;
+    std::vector<int32_t> lmax;
+    ifs_params prm{};
+    double tol = 1e-6;
+    int restart = 30, max_iter = 1000;
+    size_t footprint = 0;
+    clk::time_point t_submit, t_start, t_done;
+    // result
+    int status = 0;
+    bool done = false;
+    std::vector<double> csec;  // 3 per direction
+    int iterations = 0;
+};
+
+}  // namespace
+
+struct ifs_server {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::map<int64_t, Req> reqs;
+    std::vector<std::thread> workers;
+    size_t budget = 0, admitted = 0, peak_admitted = 0;
+    int running = 0, peak_running = 0;
+    int64_t next_id = 1;
+    bool stop = false;
+    clk::time_point t0;
+    std::vector<double> latencies;
+    int64_t completed = 0;
+};
+
+namespace {
+
+// Device bytes a request needs: vectors, Krylov basis of the lock-step GMRES and
+// the direct-path translation tables (kept for the whole request).
+size_t estimate_bytes(const Req &r) {
+    const int ns = (int)r.radii.size();
+    int N = r.prm.n_global;
+    const double k = r.prm.k;
+    std::vector<int> L(ns);
+    for (int j = 0; j < ns; ++j) {
+        int lj = r.lmax.empty() ? (int)std::ceil(k * r.radii[j] + 4.05 * std::cbrt(k * r.radii[j]) + 2.0) : r.lmax[j];
+        L[j] = N > 0 ? std::min(lj, N) : lj;
+    }
+    if (N <= 0) N = *std::max_element(L.begin(), L.end());
+    const double n = 2.0 * ns * N * (N + 2);
+    const int nd = (int)r.alpha.size();
+    double bytes = 16.0 * n * ((r.restart + 1) * nd + 6.0 * nd + 4.0);
+    double tab = 0.0;
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b < ns; ++b) {
+            if (a == b) continue;
+            double Ls = L[a], Lt = L[b], K = std::min(Ls, Lt), Lm = std::max(Ls, Lt);
+            tab += 8.0 * (2 * K + 1) * (Lm + 1) * (Lm + 1) + 16.0 * 2.0 * (2 * K + 1) * Ls * Lt +
+                   16.0 * 2.0 * (2 * K + 1) * (Ls + Lt);
+        }
+    return (size_t)(bytes + tab) + ((size_t)64 << 20);
+}
+
+void run_request(Req &r) {
+    ifs_cluster cl{(int32_t)r.radii.size(), r.centres.data(), r.radii.data(), r.mrel.data(),
+                   r.lmax.empty() ? nullptr : r.lmax.data()};
+    ifs_ctx *c = nullptr;
+    int rc = ifs_create(&c, &cl, &r.prm);
+    if (rc) {
+        r.status = rc;
+        return;
+    }
+    const int nd = (int)r.alpha.size();
+    const int64_t n = ifs_vector_length(c);
+    std::vector<double> rhs(2 * n * nd), p(2 * n * nd), x(2 * n * nd);
+    for (int d = 0; d < nd && !rc; ++d) {
+        rc = ifs_plane_wave_rhs_dir(c, r.alpha[d], r.beta[d], rhs.data() + 2 * n * d, n);
+        if (!rc) rc = ifs_plane_wave(c, r.alpha[d], r.beta[d], p.data() + 2 * n * d, n);
+    }
+    std::vector<ifs_solve_info> info(nd);
+    if (!rc) {
+        ifs_solve_opts op{r.restart, r.max_iter, r.tol, 0, {0, 0, 0}};
+        rc = ifs_solve_batch(c, nd, rhs.data(), x.data(), n, &op, info.data());
+    }
+    r.csec.assign(3 * nd, 0.0);
+    for (int d = 0; d < nd && (rc == IFS_OK || rc == IFS_E_NOCONV); ++d) {
+        int q = ifs_cross_sections(c, x.data() + 2 * n * d, p.data() + 2 * n * d, n, r.csec.data() + 3 * d);
+        if (q) rc = q;
+        r.iterations = std::max(r.iterations, info[d].iterations);
+    }
+    r.status = rc;
+    ifs_destroy(c);
+}
+
+void worker(ifs_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        // the oldest queued request that fits; the oldest one anyway if nothing runs
+        int64_t pick = 0;
+        s->cv_work.wait(lk, [&] {
+            if (s->stop && s->queue.empty()) return true;
+            for (int64_t id : s->queue) {
+                Req &r = s->reqs[id];
+                if (s->admitted + r.footprint <= s->budget || s->running == 0) {
+                    pick = id;
+                    return true;
+                }
+            }
+            return false;
+        });
+        if (!pick) {
+            if (s->stop && s->queue.empty()) return;
+            continue;
+        }
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), pick));
+        Req &r = s->reqs[pick];
+        s->admitted += r.footprint;
+        s->peak_admitted = std::max(s->peak_admitted, s->admitted);
+        s->running += 1;
+        s->peak_running = std::max(s->peak_running, s->running);
+        r.t_start = clk::now();
+        lk.unlock();
+        run_request(r);  // r stays valid: map nodes are stable and erased only by the waiter
+        lk.lock();
+        r.t_done = clk::now();
+        r.done = true;
+        s->admitted -= r.footprint;
+        s->running -= 1;
+        s->latencies.push_back(std::chrono::duration<double>(r.t_done - r.t_submit).count());
+        s->completed += 1;
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ifs_server_create(ifs_server **out, int32_t slots, double hbm_budget_gb) {
+    if (!out || slots < 1 || slots > 64 || !(hbm_budget_gb > 0.0)) return IFS_E_ARG;
+    ifs_server *s = new ifs_server();
+    s->budget = (size_t)(hbm_budget_gb * 1073741824.0);
+    s->t0 = clk::now();
+    for (int i = 0; i < slots; ++i) s->workers.emplace_back(worker, s);
+    *out = s;
+    return IFS_OK;
+}
+
+int ifs_server_submit(ifs_server *s, const ifs_cluster *cl, const ifs_params *prm, int32_t ndir, const double *alpha,
+                      const double *beta, double tol, int32_t restart, int64_t *id_out) {
+    if (!s || !cl || !prm || ndir < 1 || !alpha || !beta || !id_out || cl->n_spheres < 1 || !cl->centres ||
+        !cl->radii || !cl->m_rel || !(tol > 0.0) || restart < 1)
+        return IFS_E_ARG;
+    Req r;
+    const int ns = cl->n_spheres;
+    r.centres.assign(cl->centres, cl->centres + 3 * ns);
+    r.radii.assign(cl->radii, cl->radii + ns);
+    r.mrel.assign(cl->m_rel, cl->m_rel + 2 * ns);
+    if (cl->lmax) r.lmax.assign(cl->lmax, cl->lmax + ns);
+    r.alpha.assign(alpha, alpha + ndir);
+    r.beta.assign(beta, beta + ndir);
+    r.prm = *prm;
+    r.tol = tol;
+    r.restart = restart;
+    r.footprint = estimate_bytes(r);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return IFS_E_ARG;
+    r.id = s->next_id++;
+    r.t_submit = clk::now();
+    *id_out = r.id;
+    s->reqs.emplace(r.id, std::move(r));
+    s->queue.push_back(*id_out);
+    s->cv_work.notify_all();
+    return IFS_OK;
+}
+
+int ifs_server_wait(ifs_server *s, int64_t id, double *csec, ifs_request_info *info) {
+    if (!s) return IFS_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return IFS_E_ARG;
+    s->cv_done.wait(lk, [&] { return it->second.done; });
+    Req &r = it->second;
+    if (csec) std::copy(r.csec.begin(), r.csec.end(), csec);
+    if (info) {
+        info->status = r.status;
+        info->iterations = r.iterations;
+        info->latency_s = std::chrono::duration<double>(r.t_done - r.t_submit).count();
+        info->queue_s = std::chrono::duration<double>(r.t_start - r.t_submit).count();
+        info->footprint_bytes = (int64_t)r.footprint;
+    }
+    int st = r.status;
+    s->reqs.erase(it);
+    return st;
+}
+
+int ifs_server_stats(ifs_server *s, double out[6]) {
+    if (!s || !out) return IFS_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<double> l = s->latencies;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double q) {
+        if (l.empty()) return 0.0;
+        size_t i = (size_t)std::ceil(q * l.size()) - 1;
+        return l[std::min(i, l.size() - 1)];
+    };
+    double el = std::chrono::duration<double>(clk::now() - s->t0).count();
+    out[0] = (double)s->completed;
+    out[1] = pct(0.50);
+    out[2] = pct(0.99);
+    out[3] = el > 0.0 ? s->completed / el : 0.0;
+    out[4] = (double)s->peak_running;
+    out[5] = (double)s->peak_admitted;
+    return IFS_OK;
+}
+
+int ifs_server_destroy(ifs_server *s) {
+    if (!s) return IFS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    for (auto &t : s->workers) t.join();
+    delete s;
+    return IFS_OK;
+}
+
+}  // extern "C"

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/ifs.h"
+#include "ifs_engine.h"
 #include "ifs_internal.h"
 
 using namespace ifs;
@@ -36,7 +37,6 @@ int ctx_ns(ifs_ctx *c);
 int ctx_N(ifs_ctx *c);
 const double *ctx_pos(ifs_ctx *c);
 double ctx_k(ifs_ctx *c);
-int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised);
 int scale_flat(ifs_ctx *c, const double2 *x, double2 *y, int inverse);
 }  // namespace ifs
 
@@ -92,7 +92,7 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     }
     // r = b - M x
     auto residual = [&]() -> int {
-        int q = matvec_impl(c, x, r, 1);
+        int q = matvec_impl(c, x, r, 1, 1, n);
         if (q) return q;
         ++iters;
         launch_axpby(one, b, mone, r, n, s);
@@ -111,7 +111,7 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         int jend = 0;
         for (int j = 0; j < m; ++j) {
             double2 *vj = V + (int64_t)j * n;
-            if ((rc = matvec_impl(c, vj, w, 1))) return rc;
+            if ((rc = matvec_impl(c, vj, w, 1, 1, n))) return rc;
             ++iters;
             // CGS with one re-orthogonalisation pass
             if ((rc = dots(s, V, n, j + 1, w, n, h, part, hv))) return rc;
@@ -188,6 +188,200 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     }
     if (converged) return IFS_OK;
     return breakdown ? IFS_E_BREAKDOWN : IFS_E_NOCONV;
+}
+
+// Restarted GMRES for nrhs right-hand sides in lock step: every Arnoldi step applies
+// the operator to the current Krylov vector of all right-hand sides in one batched
+// matvec (the translation tables serve every column); orthogonalisation (CGS2),
+// Givens rotations and the true-residual test stay per right-hand side.  A right-
+// hand side that converged keeps riding along in the batched matvec until all are
+// done, and its iterate no longer changes.
+extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n,
+                                      int64_t ld, const ifs_solve_opts *opts, ifs_solve_info *info) {
+    if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c) || nrhs < 1 || ld < n) return IFS_E_ARG;
+    const int m = opts->restart;
+    if (m < 1 || opts->max_iter < 1 || !(opts->tol > 0.0)) return IFS_E_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = ctx_stream(c);
+    DevBuf dV, dW, dR, dB, dX, dpart, dh;
+    int rc;
+    const int64_t ldv = (int64_t)(m + 1) * n;  // V[r][j][n]
+    if ((rc = dV.alloc(16ull * ldv * nrhs)) || (rc = dW.alloc(16ull * n * nrhs)) || (rc = dR.alloc(16ull * n * nrhs)) ||
+        (rc = dB.alloc(16ull * n * nrhs)) || (rc = dX.alloc(16ull * n * nrhs)) ||
+        (rc = dpart.alloc(16ull * kDotBlocks * (m + 1))) || (rc = dh.alloc(16ull * (m + 1))))
+        return rc;
+    double2 *V = dV.as<double2>(), *W = dW.as<double2>(), *R = dR.as<double2>(), *B = dB.as<double2>(),
+            *X = dX.as<double2>(), *part = dpart.as<double2>(), *h = dh.as<double2>();
+    launch_zero(V, ldv * nrhs, s);
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
+    const double2 *bin = (const double2 *)rhs_dev;
+    double2 *xout = (double2 *)x_dev;
+    std::vector<double> bnorm(nrhs), beta(nrhs);
+    std::vector<int> iters(nrhs, 0), conv(nrhs, 0), brk(nrhs, 0);
+    for (int r = 0; r < nrhs; ++r) {
+        if ((rc = scale_flat(c, bin + r * ld, B + r * n, 0))) return rc;
+        if (opts->use_x0) {
+            if ((rc = scale_flat(c, xout + r * ld, X + r * n, 0))) return rc;
+        } else {
+            launch_zero(X + r * n, n, s);
+        }
+        bnorm[r] = norm2(s, B + r * n, n, h, part, rc);
+        if (rc) return rc;
+    }
+    int total_mv = 0;
+    auto residual = [&]() -> int {
+        int q = matvec_impl(c, X, R, 1, nrhs, n);
+        if (q) return q;
+        ++total_mv;
+        for (int r = 0; r < nrhs; ++r) {
+            if (conv[r]) continue;
+            ++iters[r];
+            launch_axpby(one, B + r * n, mone, R + r * n, n, s);
+            beta[r] = norm2(s, R + r * n, n, h, part, q);
+            if (q) return q;
+            conv[r] = bnorm[r] == 0.0 || beta[r] / bnorm[r] < opts->tol;
+        }
+        return IFS_OK;
+    };
+    if ((rc = residual())) return rc;
+    struct St {
+        std::vector<cplx> H, g, cs, sn;
+        int jend = 0;
+        bool done = false;
+    };
+    std::vector<St> st(nrhs);
+    auto active = [&](int r) { return !conv[r] && !brk[r] && iters[r] < opts->max_iter; };
+    std::vector<cplx> hv, hv2;
+    for (;;) {
+        bool any = false;
+        for (int r = 0; r < nrhs; ++r) any = any || active(r);
+        if (!any) break;
+        for (int r = 0; r < nrhs; ++r) {
+            St &S = st[r];
+            S.done = !active(r);
+            S.jend = 0;
+            if (S.done) continue;
+            S.H.assign((size_t)(m + 1) * m, 0.0);
+            S.g.assign(m + 1, 0.0);
+            S.cs.assign(m, 0.0);
+            S.sn.assign(m, 0.0);
+            S.g[0] = beta[r];
+            launch_scale_copy(make_double2(1.0 / beta[r], 0.0), R + r * n, V + r * ldv, n, s);
+        }
+        for (int j = 0; j < m; ++j) {
+            bool work = false;
+            for (int r = 0; r < nrhs; ++r) work = work || !st[r].done;
+            if (!work) break;
+            // w_r = M v_{r,j} for every right-hand side in one call
+            if ((rc = matvec_impl(c, V + (int64_t)j * n, W, 1, nrhs, ldv, n))) return rc;
+            ++total_mv;
+            for (int r = 0; r < nrhs; ++r) {
+                St &S = st[r];
+                if (S.done) continue;
+                ++iters[r];
+                double2 *Vr = V + r * ldv, *w = W + r * n;
+                if ((rc = dots(s, Vr, n, j + 1, w, n, h, part, hv))) return rc;
+                if (hipMemcpyAsync(h, hv.data(), sizeof(double2) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+                    return IFS_E_HIP;
+                launch_gemv_sub(Vr, n, j + 1, h, w, n, s);
+                if ((rc = dots(s, Vr, n, j + 1, w, n, h, part, hv2))) return rc;
+                if (hipMemcpyAsync(h, hv2.data(), sizeof(double2) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+                    return IFS_E_HIP;
+                launch_gemv_sub(Vr, n, j + 1, h, w, n, s);
+                for (int i = 0; i <= j; ++i) hv[i] += hv2[i];
+                double hn = norm2(s, w, n, h, part, rc);
+                if (rc) return rc;
+                std::vector<cplx> &H = S.H;
+                for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i];
+                H[(size_t)(j + 1) * m + j] = hn;
+                for (int i = 0; i < j; ++i) {
+                    cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                    H[(size_t)i * m + j] = S.cs[i] * a + S.sn[i] * bb;
+                    H[(size_t)(i + 1) * m + j] = -std::conj(S.sn[i]) * a + S.cs[i] * bb;
+                }
+                cplx a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+                double ra = std::abs(a), rb = std::abs(bb), rr = std::hypot(ra, rb);
+                if (rr == 0.0) {
+                    S.cs[j] = 1.0;
+                    S.sn[j] = 0.0;
+                } else if (ra == 0.0) {
+                    S.cs[j] = 0.0;
+                    S.sn[j] = std::conj(bb) / rb;
+                    H[(size_t)j * m + j] = rb;
+                } else {
+                    S.cs[j] = ra / rr;
+                    S.sn[j] = (a / ra) * std::conj(bb) / rr;
+                    H[(size_t)j * m + j] = (a / ra) * rr;
+                }
+                H[(size_t)(j + 1) * m + j] = 0.0;
+                S.g[j + 1] = -std::conj(S.sn[j]) * S.g[j];
+                S.g[j] = S.cs[j] * S.g[j];
+                S.jend = j + 1;
+                double est = std::abs(S.g[j + 1]) / bnorm[r];
+                if (hn == 0.0) {
+                    brk[r] = 1;
+                    S.done = true;
+                } else if (est < opts->tol || iters[r] >= opts->max_iter || j + 1 == m) {
+                    S.done = true;
+                } else {
+                    launch_scale_copy(make_double2(1.0 / hn, 0.0), w, Vr + (int64_t)(j + 1) * n, n, s);
+                }
+            }
+        }
+        for (int r = 0; r < nrhs; ++r) {
+            St &S = st[r];
+            if (S.jend == 0) continue;
+            std::vector<cplx> y(S.jend);
+            for (int i = S.jend - 1; i >= 0; --i) {
+                cplx acc = S.g[i];
+                for (int k = i + 1; k < S.jend; ++k) acc -= S.H[(size_t)i * m + k] * y[k];
+                y[i] = acc / S.H[(size_t)i * m + i];
+            }
+            if (hipMemcpyAsync(h, y.data(), sizeof(double2) * S.jend, hipMemcpyHostToDevice, s) != hipSuccess)
+                return IFS_E_HIP;
+            launch_gemv_add(V + r * ldv, n, S.jend, h, X + r * n, n, s);
+        }
+        if ((rc = residual())) return rc;
+        for (int r = 0; r < nrhs; ++r)
+            if (brk[r] && conv[r]) brk[r] = 0;
+    }
+    for (int r = 0; r < nrhs; ++r)
+        if ((rc = scale_flat(c, X + r * n, xout + r * ld, 1))) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return IFS_E_HIP;
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int all = 1, anybrk = 0;
+    for (int r = 0; r < nrhs; ++r) {
+        if (info) {
+            info[r].iterations = iters[r];
+            info[r].converged = conv[r];
+            info[r].rel_residual = bnorm[r] > 0.0 ? beta[r] / bnorm[r] : 0.0;
+            info[r].seconds = secs;
+        }
+        all = all && conv[r];
+        anybrk = anybrk || brk[r];
+    }
+    (void)total_mv;
+    if (all) return IFS_OK;
+    return anybrk ? IFS_E_BREAKDOWN : IFS_E_NOCONV;
+}
+
+extern "C" int ifs_solve_batch(ifs_ctx *c, int32_t nrhs, const double *rhs, double *x, int64_t n,
+                               const ifs_solve_opts *opts, ifs_solve_info *info) {
+    if (!c || !rhs || !x || !opts || n != ifs_vector_length(c) || nrhs < 1) return IFS_E_ARG;
+    hipStream_t s = ctx_stream(c);
+    DevBuf db, dx;
+    int rc;
+    if ((rc = db.alloc(16ull * n * nrhs)) || (rc = dx.alloc(16ull * n * nrhs))) return rc;
+    if (hipMemcpyAsync(db.p, rhs, 16ull * n * nrhs, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (opts->use_x0 && hipMemcpyAsync(dx.p, x, 16ull * n * nrhs, hipMemcpyHostToDevice, s) != hipSuccess))
+        return IFS_E_HIP;
+    rc = ifs_solve_batch_device(c, nrhs, db.p, dx.p, n, n, opts, info);
+    if (rc == IFS_OK || rc == IFS_E_NOCONV || rc == IFS_E_BREAKDOWN) {
+        if (hipMemcpyAsync(x, dx.p, 16ull * n * nrhs, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return IFS_E_HIP;
+    }
+    return rc;
 }
 
 extern "C" int ifs_solve(ifs_ctx *c, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,

@@ -16,7 +16,8 @@ from ._lib import check, lib
 
 class MultiSphereSolver:
     def __init__(self, centres, radii, m_rel, k, n_global=None, lmax=None, z_selector=3, ref_row_zeroing=False,
-                 use_fmm=False, leaf_size=32, fmm_d0=6.0, fmm_lmax_cap=0):
+                 use_fmm=False, leaf_size=32, fmm_d0=6.0, fmm_lmax_cap=0, workspace_mb=0,
+                 d_convention="physical"):
         centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
         radii = np.ascontiguousarray(radii, dtype=np.float64)
         ns = radii.shape[0]
@@ -38,6 +39,10 @@ class MultiSphereSolver:
         pr.leaf_size = int(leaf_size)
         pr.fmm_d0 = float(fmm_d0)
         pr.fmm_lmax_cap = int(fmm_lmax_cap)
+        pr.workspace_mb = int(workspace_mb)
+        if d_convention not in ("physical", "reference"):
+            raise ValueError("d_convention must be 'physical' or 'reference'")
+        pr.d_convention = 0 if d_convention == "physical" else 1
         h = ctypes.c_void_p()
         check(lib().ifs_create(ctypes.byref(h), ctypes.byref(cl), ctypes.byref(pr)), "ifs_create")
         self._h = h
@@ -68,6 +73,60 @@ class MultiSphereSolver:
         check(lib().ifs_plane_wave_rhs(self._h, rhs.ctypes.data, self.n), "ifs_plane_wave_rhs")
         return rhs
 
+    def plane_wave(self, alpha=0.0, beta=0.0):
+        """Incident coefficients (p_mn, q_mn) of a unit plane wave along (alpha, beta)
+        (get_p_q_j_j_plane_wave_core, lib_mie_illumination.f90:636-763)."""
+        p = np.empty(self.n, dtype=np.complex128)
+        check(lib().ifs_plane_wave(self._h, float(alpha), float(beta), p.ctypes.data, self.n), "ifs_plane_wave")
+        return p
+
+    def get_vector_b_dir(self, alpha=0.0, beta=0.0):
+        """Right-hand side T p for a plane wave along (alpha, beta)."""
+        rhs = np.empty(self.n, dtype=np.complex128)
+        check(lib().ifs_plane_wave_rhs_dir(self._h, float(alpha), float(beta), rhs.ctypes.data, self.n),
+              "ifs_plane_wave_rhs_dir")
+        return rhs
+
+    def cross_sections(self, x, p):
+        """(C_sca, C_ext, C_abs) of the cluster from solved coefficients x and the incident p."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        p = np.ascontiguousarray(p, dtype=np.complex128)
+        out = np.zeros(3)
+        check(lib().ifs_cross_sections(self._h, x.ctypes.data, p.ctypes.data, self.n, out.ctypes.data),
+              "ifs_cross_sections")
+        return tuple(out)
+
+    def far_field(self, x, theta, phi):
+        """(F_theta, F_phi) at the directions (theta, phi): E_s ~ e^{ikr}/(kr) F."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        ph = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(phi), th.shape), dtype=np.float64)
+        out = np.empty(2 * th.size, dtype=np.complex128)
+        check(lib().ifs_far_field(self._h, x.ctypes.data, self.n, th.size, th.ctypes.data, ph.ctypes.data,
+                                  out.ctypes.data), "ifs_far_field")
+        return out[0::2], out[1::2]
+
+    @staticmethod
+    def amplitude_matrix(phi, Fx, Fy):
+        """S1..S4 from the far fields of the x- and y-polarised waves along +z."""
+        ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+        fx = np.ascontiguousarray(np.stack([Fx[0], Fx[1]], axis=1).ravel(), dtype=np.complex128)
+        fy = np.ascontiguousarray(np.stack([Fy[0], Fy[1]], axis=1).ravel(), dtype=np.complex128)
+        S = np.empty(4 * ph.size, dtype=np.complex128)
+        check(lib().ifs_amplitude_matrix(ph.size, ph.ctypes.data, fx.ctypes.data, fy.ctypes.data, S.ctypes.data),
+              "ifs_amplitude_matrix")
+        S = S.reshape(-1, 4)
+        return S[:, 0], S[:, 1], S[:, 2], S[:, 3]
+
+    @staticmethod
+    def mueller(S1, S2, S3, S4):
+        """Mueller matrices [ndir, 4, 4]."""
+        S = np.ascontiguousarray(np.stack([S1, S2, S3, S4], axis=1).ravel(), dtype=np.complex128)
+        nd = S.size // 4
+        M = np.empty(16 * nd)
+        check(lib().ifs_mueller(nd, S.ctypes.data, M.ctypes.data), "ifs_mueller")
+        return M.reshape(nd, 4, 4)
+
     def mie_coefficients(self):
         """(a_n^j, b_n^j) as arrays [n_spheres, N] (lib_mie_type_func_get_sphere_parameter)."""
         an = np.empty((self.n_spheres, self.n_global), dtype=np.complex128)
@@ -86,6 +145,28 @@ class MultiSphereSolver:
             check(rc, "ifs_solve")
         return x, dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
                        seconds=inf.seconds)
+
+    def calculate_vector_b_batch(self, X):
+        """b_r = (I + T A) x_r for the rows x_r of X [nrhs, n] in one batched pass."""
+        X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.complex128)
+        if X.shape[1] != self.n:
+            raise ValueError(f"X has shape {X.shape}, expected (nrhs, {self.n})")
+        B = np.empty_like(X)
+        check(lib().ifs_matvec_batch(self._h, X.shape[0], X.ctypes.data, B.ctypes.data, self.n), "ifs_matvec_batch")
+        return B
+
+    def solve_batch(self, RHS, restart=50, max_iter=1000, tol=1e-6, raise_on_noconv=True):
+        """Lock-step GMRES for the rows of RHS [nrhs, n]; returns (X, [info dicts])."""
+        Bm = np.ascontiguousarray(np.atleast_2d(RHS), dtype=np.complex128)
+        nr = Bm.shape[0]
+        X = np.zeros_like(Bm)
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), 0)
+        infos = (_lib.SolveInfo * nr)()
+        rc = lib().ifs_solve_batch(self._h, nr, Bm.ctypes.data, X.ctypes.data, self.n, ctypes.byref(op), infos)
+        if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
+            check(rc, "ifs_solve_batch")
+        return X, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
+                        seconds=i.seconds) for i in infos]
 
     def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False):
         op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)))

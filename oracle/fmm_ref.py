@@ -13,7 +13,7 @@ C++ code:
 
 The MLFMM passes follow libmlfmm (libmlfmm/src/lib/lib_ml_fmm.f90:231-1204) with
 full Xu translation matrices from the C oracle (Gaunt / 3j formula) in Xu's basis,
-translation vector d = origin(source) - origin(target) throughout, j_n for
+translation vector d = dsign * (origin(source) - origin(target)) throughout, j_n for
 outgoing->outgoing and regular->regular, h1_n for outgoing->regular.
 """
 import math
@@ -124,16 +124,19 @@ def _translate(A, B, xa, xb):
     return A @ xa + B @ xb, B @ xa + A @ xb
 
 
-def _T(src, dst, k, Ls, Lt, zsel):
-    d = np.asarray(src, dtype=np.float64) - np.asarray(dst, dtype=np.float64)
+def _T(src, dst, k, Ls, Lt, zsel, dsign=1.0):
+    d = dsign * (np.asarray(src, dtype=np.float64) - np.asarray(dst, dtype=np.float64))
     r = float(np.linalg.norm(d))
     th = math.acos(max(-1.0, min(1.0, d[2] / r))) if r > 0 else 0.0
     ph = math.atan2(d[1], d[0]) if (d[0] != 0.0 or d[1] != 0.0) else 0.0
     return O.translation_ab(k * r, th, ph, Lt, Ls, zsel)
 
 
-def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3):
-    """b = x + T (far field + near field) x in the reference flat layout."""
+def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3, dsign=1.0):
+    """b = x + T (far field + near field) x in the reference flat layout.
+
+    dsign = +1: d = origin(source) - origin(target) as the reference's direct path
+    passes it; dsign = -1: the opposite sign."""
     pos = np.asarray(pos, dtype=np.float64)
     ns = len(pos)
     tr = build_tree(pos, s)
@@ -160,7 +163,7 @@ def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3):
     # P2M
     for i in range(ns):
         b = tr["leaf"][i]
-        A, B = _T(pos[i], LL["centre"][b], k, Leff[i], LL["L"], 1)
+        A, B = _T(pos[i], LL["centre"][b], k, Leff[i], LL["L"], 1, dsign)
         ya, yb = _translate(A, B, *xs(i))
         C[lmax][b][0] += ya
         C[lmax][b][1] += yb
@@ -168,7 +171,7 @@ def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3):
     for l in range(lmax - 1, lmin - 1, -1):
         for b in boxes[l + 1]["boxes"]:
             p = b >> 3
-            A, B = _T(boxes[l + 1]["centre"][b], boxes[l]["centre"][p], k, boxes[l + 1]["L"], boxes[l]["L"], 1)
+            A, B = _T(boxes[l + 1]["centre"][b], boxes[l]["centre"][p], k, boxes[l + 1]["L"], boxes[l]["L"], 1, dsign)
             ya, yb = _translate(A, B, *C[l + 1][b])
             C[l][p][0] += ya
             C[l][p][1] += yb
@@ -180,7 +183,7 @@ def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3):
             for src in i4(b, l):
                 if src not in present:
                     continue
-                A, B = _T(V["centre"][src], V["centre"][b], k, V["L"], V["L"], zsel)
+                A, B = _T(V["centre"][src], V["centre"][b], k, V["L"], V["L"], zsel, dsign)
                 ya, yb = _translate(A, B, *C[l][src])
                 Dl[l][b][0] += ya
                 Dl[l][b][1] += yb
@@ -188,7 +191,7 @@ def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3):
             P = boxes[l - 1]
             for b in V["boxes"]:
                 p = b >> 3
-                A, B = _T(P["centre"][p], V["centre"][b], k, P["L"], V["L"], 1)
+                A, B = _T(P["centre"][p], V["centre"][b], k, P["L"], V["L"], 1, dsign)
                 ya, yb = _translate(A, B, *Dl[l - 1][p])
                 Dl[l][b][0] += ya
                 Dl[l][b][1] += yb
@@ -199,14 +202,14 @@ def fmm_matvec(pos, lmax_j, nglob, an, bn, k, x, s=32, d0=6.0, cap=0, zsel=3):
         members.setdefault(tr["leaf"][i], []).append(i)
     for j in range(ns):
         b = tr["leaf"][j]
-        A, B = _T(LL["centre"][b], pos[j], k, LL["L"], Leff[j], 1)
+        A, B = _T(LL["centre"][b], pos[j], k, LL["L"], Leff[j], 1, dsign)
         va, vb = _translate(A, B, *Dl[lmax][b])
         near = set(adjacent(b, lmax)) | {b}
         for nb in sorted(near):
             for i in members.get(nb, []):
                 if i == j:
                     continue
-                A, B = _T(pos[i], pos[j], k, Leff[i], Leff[j], zsel)
+                A, B = _T(pos[i], pos[j], k, Leff[i], Leff[j], zsel, dsign)
                 ya, yb = _translate(A, B, *xs(i))
                 va = va + ya
                 vb = vb + yb

@@ -11,6 +11,10 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10  # FP64 mode: relative 2-norm error vs the oracle
 
+# The oracle restates the reference's direct path, d = origin(source) - origin(target).
+# The default (physical) convention is d = origin(target) - origin(source); the oracle
+# evaluates it at the mirrored positions -pos (it sees positions only through d).
+
 
 def _rand(n, seed):
     rng = np.random.default_rng(seed)
@@ -25,18 +29,20 @@ def _small_cluster(ns, lmax_hi, seed):
     return pos, radii, lmax
 
 
+@pytest.mark.parametrize("conv", ["physical", "reference"])
 @pytest.mark.parametrize("zsel", [3, 1])
 @pytest.mark.parametrize("ns,lhi,seed", [(2, 3, 0), (5, 6, 1), (7, 9, 2)])
-def test_direct_matvec_small(oracle, zsel, ns, lhi, seed):
+def test_direct_matvec_small(oracle, conv, zsel, ns, lhi, seed):
     pos, radii, lmax = _small_cluster(ns, lhi, seed)
     k = 1.3
-    s = MultiSphereSolver(pos, radii, 1.5 + 0.01j, k, lmax=lmax, z_selector=zsel)
+    s = MultiSphereSolver(pos, radii, 1.5 + 0.01j, k, lmax=lmax, z_selector=zsel, d_convention=conv)
     an, bn = s.mie_coefficients()
     x = _rand(s.n, seed + 10)
     b = s.calculate_vector_b(x)
-    ref = oracle.matvec_direct(pos, lmax, s.n_global, an, bn, k, x, zsel=zsel)
+    opos = -pos if conv == "physical" else pos
+    ref = oracle.matvec_direct(opos, lmax, s.n_global, an, bn, k, x, zsel=zsel)
     err = np.linalg.norm(b - ref) / np.linalg.norm(ref)
-    print(f"ns={ns} lhi={lhi} zsel={zsel} rel err {err:.3e}")
+    print(f"ns={ns} lhi={lhi} zsel={zsel} {conv} rel err {err:.3e}")
     assert err < RTOL
     s.close()
 
@@ -59,7 +65,7 @@ def test_direct_matvec_config1(oracle):
     an, bn = s.mie_coefficients()
     x = _rand(s.n, 7)
     b = s.calculate_vector_b(x)
-    ref = oracle.matvec_direct(c["centres"], c["lmax"], s.n_global, an, bn, c["k"], x, zsel=3)
+    ref = oracle.matvec_direct(-c["centres"], c["lmax"], s.n_global, an, bn, c["k"], x, zsel=3)
     err = np.linalg.norm(b - ref) / np.linalg.norm(ref)
     print(f"config1 rel err {err:.3e}")
     assert err < RTOL
@@ -69,7 +75,7 @@ def test_direct_matvec_config1(oracle):
 def test_ref_row_zeroing(oracle):
     pos, radii, lmax = _small_cluster(5, 5, 3)
     k = 1.1
-    s = MultiSphereSolver(pos, radii, 1.4, k, lmax=lmax, ref_row_zeroing=True)
+    s = MultiSphereSolver(pos, radii, 1.4, k, lmax=lmax, ref_row_zeroing=True, d_convention="reference")
     an, bn = s.mie_coefficients()
     x = _rand(s.n, 4)
     b = s.calculate_vector_b(x)
@@ -87,7 +93,7 @@ def test_solve_config1(oracle):
     print(info)
     assert info["converged"]
     an, bn = s.mie_coefficients()
-    Mx = oracle.matvec_direct(c["centres"], c["lmax"], s.n_global, an, bn, c["k"], x, zsel=3)
+    Mx = oracle.matvec_direct(-c["centres"], c["lmax"], s.n_global, an, bn, c["k"], x, zsel=3)
     # residual in the normalised basis s(n,m) = sqrt(n(n+1)(2n+1)(n-m)!/(n+m)!)
     N = s.n_global
     snm = np.array([np.sqrt(n * (n + 1) * (2 * n + 1) * math.factorial(n - m) / math.factorial(n + m))

@@ -13,19 +13,21 @@ def _rand(n, seed):
     return rng.standard_normal(n) + 1j * rng.standard_normal(n)
 
 
-@pytest.mark.parametrize("ns,s,d0,cap,seed", [(24, 2, 6.0, 0, 11), (40, 1, 3.0, 0, 12), (30, 2, 6.0, 5, 13)])
-def test_fmm_vs_oracle_fmm(oracle, ns, s, d0, cap, seed):
+@pytest.mark.parametrize("ns,s,d0,cap,seed,conv", [(24, 2, 6.0, 0, 11, "physical"), (40, 1, 3.0, 0, 12, "physical"),
+                                                    (30, 2, 6.0, 5, 13, "physical"), (24, 2, 6.0, 0, 14, "reference")])
+def test_fmm_vs_oracle_fmm(oracle, ns, s, d0, cap, seed, conv):
     from oracle import fmm_ref
     rng = np.random.default_rng(seed)
     pos = rng.uniform(0.0, 8.0, (ns, 3))
     lmax = rng.integers(1, 4, ns).astype(np.int32)
     k = 0.6
     solver = MultiSphereSolver(pos, np.full(ns, 0.2), 1.5 + 0.01j, k, lmax=lmax, use_fmm=True, leaf_size=s,
-                               fmm_d0=d0, fmm_lmax_cap=cap)
+                               fmm_d0=d0, fmm_lmax_cap=cap, d_convention=conv)
     an, bn = solver.mie_coefficients()
     x = _rand(solver.n, seed)
     b = solver.calculate_vector_b(x)
-    ref, tr = fmm_ref.fmm_matvec(pos, lmax, solver.n_global, an, bn, k, x, s=s, d0=d0, cap=cap)
+    dsign = -1.0 if conv == "physical" else 1.0
+    ref, tr = fmm_ref.fmm_matvec(pos, lmax, solver.n_global, an, bn, k, x, s=s, d0=d0, cap=cap, dsign=dsign)
     u, lmin, lm = solver.tree_export()
     assert (lmin, lm) == (tr["lmin"], tr["lmax"])
     assert list(u) == tr["leaf"]
@@ -49,3 +51,21 @@ def test_fmm_vs_direct_config1():
     assert err < 1e-4
     d.close()
     f.close()
+
+
+def test_fmm_chunked_workspace_config1():
+    """A 1 MiB translation workspace cuts the level classes into many chunks (class
+    columns split across chunks, rotation runs and axial segments cut at chunk
+    borders); the matvec must not change beyond summation order."""
+    c = config_cluster(1)
+    kw = dict(lmax=c["lmax"], use_fmm=True, leaf_size=2, fmm_d0=6.0)
+    a = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
+    b = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], workspace_mb=1, **kw)
+    x = _rand(a.n, 5)
+    ya = a.calculate_vector_b(x)
+    yb = b.calculate_vector_b(x)
+    err = np.linalg.norm(ya - yb) / np.linalg.norm(ya - x)
+    print("chunked vs single-chunk workspace", err, a.fmm_stats())
+    assert err < 1e-13
+    a.close()
+    b.close()
