@@ -152,7 +152,7 @@ def main():
         rhs = torch.from_numpy(s.get_vector_b()).to("cuda")
         xs = torch.zeros_like(rhs)
         with torch.cuda.stream(st):
-            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=400, tol=1e-6)
+            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=400, tol=1e-6, verbose=True)
         out["solve"] = info
         out["solves_per_s"] = 1.0 / info["seconds"] if info["converged"] else None
     if not args.no_cpu:

@@ -40,7 +40,7 @@ class Params(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("restart", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
-                ("use_x0", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("use_x0", ctypes.c_int32), ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class RequestInfo(ctypes.Structure):

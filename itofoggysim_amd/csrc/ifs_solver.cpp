@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -174,6 +175,11 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         beta = norm2(s, r, n, h, part, rc);
         if (rc) return rc;
         converged = beta / bnorm < opts->tol;
+        if (opts->verbose) {
+            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::fprintf(stderr, "gmres: %d matvecs, true rel residual %.3e, %.1f s\n", iters, beta / bnorm, el);
+            std::fflush(stderr);
+        }
         if (breakdown && !converged) break;
         (void)zero;
     }

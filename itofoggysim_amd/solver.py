@@ -168,8 +168,8 @@ class MultiSphereSolver:
         return X, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
                         seconds=i.seconds) for i in infos]
 
-    def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False):
-        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)))
+    def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False, verbose=False):
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)), int(bool(verbose)))
         inf = _lib.SolveInfo()
         rc = lib().ifs_solve_device(self._h, ctypes.c_void_p(rhs_ptr), ctypes.c_void_p(x_ptr), self.n,
                                     ctypes.byref(op), ctypes.byref(inf))
