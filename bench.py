@@ -53,6 +53,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also run GMRES(50) to 1e-6")
+    ap.add_argument("--solve-max-iter", type=int, default=400)
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
@@ -152,7 +153,8 @@ def main():
         rhs = torch.from_numpy(s.get_vector_b()).to("cuda")
         xs = torch.zeros_like(rhs)
         with torch.cuda.stream(st):
-            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=400, tol=1e-6, verbose=True)
+            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=args.solve_max_iter, tol=1e-6,
+                                  verbose=True)
         out["solve"] = info
         out["solves_per_s"] = 1.0 / info["seconds"] if info["converged"] else None
     if not args.no_cpu:

@@ -146,7 +146,7 @@ typedef struct {
     int32_t max_iter;     /* maximum matvecs */
     double tol;           /* relative residual ||b - M x|| / ||b|| */
     int32_t use_x0;       /* 1: x holds the initial guess on entry */
-    int32_t verbose;      /* 1: one line per restart (matvecs, true residual) on stderr */
+    int32_t verbose;      /* 1: progress lines on stderr (every 5 matvecs and every restart) */
     int32_t reserved[2];
 } ifs_solve_opts;
 

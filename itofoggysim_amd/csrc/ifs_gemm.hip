@@ -378,8 +378,10 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     const int NTOT = C.ncols * C.nrhs;                     // columns of the segment
     const double2 *Pb = wax + C.off_ax + ax_off(k, Lt, Ls, K);
     const double2 *Qb = Pb + ax_block(k, Lt, Ls);
-    __shared__ double2 Ps[AK][AM];
-    __shared__ double2 Qs[AK][AM];
+    // rows padded by one entry: the tile store writes 8 inner columns of one row from
+    // 8 consecutive lanes, which a 1 KiB row stride would put on the same banks
+    __shared__ double2 Ps[AK][AM + 1];
+    __shared__ double2 Qs[AK][AM + 1];
     __shared__ double Ys[AK][4 * AN + 2];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -398,6 +400,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
         }
     double2 pa[2], pb[2];
     double py[8];
+    const bool ycol_ok = col0 + (tid >> 2) < NTOT;      // this thread's translation column
+    const double *ybase = wy + cb + 4 * (int64_t)col0 + tid;
     auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -408,18 +412,15 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             pa[e] = ok ? Pb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
             pb[e] = ok ? Qb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
         }
+        // 8 inner rows nu = q + k0 + e (one per e) x 256 reals (64 translations x 4);
+        // the row of (nu, k) advances by 2 nu + 2 (nu < K) or 2 K + 1 per degree
+        int nu = q + k0;
+        int64_t row = rows_off(nu, K) + k + (nu < K ? nu : K);
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            int idx = tid + 256 * e;  // 8 inner x 256 reals (64 translations x 4)
-            int c = idx >> 8, rr = idx & 255;
-            int nu = q + k0 + c;
-            int j = col0 + (rr >> 2);
-            double v = 0.0;
-            if (nu <= Ls && j < NTOT) {
-                int kq = nu < K ? nu : K;
-                v = wy[(rows_off(nu, K) + (k + kq)) * (int64_t)NC + cb + 4 * (int64_t)col0 + rr];
-            }
-            py[e] = v;
+            py[e] = (ycol_ok && nu <= Ls) ? ybase[row * NC] : 0.0;
+            row += nu < K ? 2 * nu + 2 : 2 * K + 1;
+            ++nu;
         }
     };
     load(0);

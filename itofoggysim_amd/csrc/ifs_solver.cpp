@@ -153,6 +153,11 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
             g[j] = cs[j] * g[j];
             jend = j + 1;
             double est = std::abs(g[j + 1]) / bnorm;
+            if (opts->verbose && iters % 5 == 0) {
+                double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::fprintf(stderr, "gmres: %d matvecs, estimated rel residual %.3e, %.1f s\n", iters, est, el);
+                std::fflush(stderr);
+            }
             if (hn == 0.0) {
                 breakdown = 1;
                 break;
