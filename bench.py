@@ -120,9 +120,19 @@ def main():
     tot_ms, launches, fl = kern[dom]
     achieved = fl / (tot_ms * 1e-3) / 1e12
     # FP64 dense products; MI355X's FP64 matrix and vector peaks are both 78.6 TFLOP/s
+    # HBM bytes per launch of the dominant kernel: measured offline with FETCH_SIZE / WRITE_SIZE
+    # counter runs of this same workload, calibrated on a plain copy (profiles/)
+    traffic, traffic_src = None, None
+    tfile = os.path.join(ROOT, "profiles", "mlfmm_2000_spheres_traffic.json")
+    if os.path.exists(tfile) and args.n == 2000:
+        tj = json.load(open(tfile))
+        t = tj["bytes_per_dispatch"].get(dom)
+        if t:
+            traffic, traffic_src = t["read"] + t["write"], "profiles/mlfmm_2000_spheres_traffic.json"
     roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                "traffic": None, "launches": launches, "avg_launch_ms": round(tot_ms / launches, 4),
+                "traffic": traffic, "traffic_unit": "bytes/launch", "traffic_source": traffic_src,
+                "launches": launches, "avg_launch_ms": round(tot_ms / launches, 4),
                 "algorithmic_flops_per_launch": fl / launches,
                 "matvec_algorithmic_tflops": round(flops_mv / (ms * 1e-3) / 1e12, 3)}
 
@@ -147,6 +157,7 @@ def main():
             "matvec_algorithmic_flop": flops_mv, "build_s": round(build_s, 3)},
         "roofline": roofline,
         "kernels_ms": {k: round(v[0], 2) for k, v in prof.items()},
+        "kernels_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) for k, v in kern.items() if v[0] > 0},
         "finite": finite,
     }
     if args.solve:

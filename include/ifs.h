@@ -67,7 +67,10 @@ typedef struct {
     int32_t workspace_mb;    /* translation workspace limit in MiB (0: min(free/4, 48 GiB)); small
                                 values split the translation classes into more workspace chunks */
     int32_t d_convention;    /* IFS_D_PHYSICAL (default) or IFS_D_REFERENCE */
-    int32_t reserved[2];
+    int32_t cache_mb;        /* device memory for translation tables kept across matvecs, in MiB
+                                (0: 40% of free memory at creation; < 0: none, tables are
+                                regenerated in every matvec) */
+    int32_t reserved;
 } ifs_params;
 
 /* Builds device tables, the Mie T-matrix and (MLFMM) the octree.

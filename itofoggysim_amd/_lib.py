@@ -35,7 +35,7 @@ class Params(ctypes.Structure):
     _fields_ = [("k", ctypes.c_double), ("n_global", ctypes.c_int32), ("z_selector", ctypes.c_int32),
                 ("ref_row_zeroing", ctypes.c_int32), ("use_fmm", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
                 ("fmm_d0", ctypes.c_double), ("fmm_lmax_cap", ctypes.c_int32), ("workspace_mb", ctypes.c_int32),
-                ("d_convention", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("d_convention", ctypes.c_int32), ("cache_mb", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class SolveOpts(ctypes.Structure):

@@ -82,10 +82,15 @@ static void set_prof_names(Batch &B) {
 }
 
 // Algorithmic flops of one generic task: rotation forward, coaxial shift, rotation back.
+// The rotations use the folded tables: E (kk+1)x(n+1) and O kk x n per degree.
 static void task_flops(const Task &T, double &ffwd, double &fax, double &fback) {
     ffwd = fax = fback = 0.0;
-    for (int n = 1; n <= T.Ls; ++n) ffwd += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
-    for (int n = 1; n <= T.Lt; ++n) fback += 8.0 * (2 * n + 1) * (2 * std::min(n, T.K) + 1);
+    auto rot = [&](int n) {
+        double kk = std::min(n, T.K);
+        return 8.0 * ((kk + 1.0) * (n + 1.0) + kk * n);
+    };
+    for (int n = 1; n <= T.Ls; ++n) ffwd += rot(n);
+    for (int n = 1; n <= T.Lt; ++n) fback += rot(n);
     for (int k = -T.K; k <= T.K; ++k) fax += 16.0 * (double)ax_block(k, T.Lt, T.Ls);
 }
 
@@ -99,7 +104,7 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
     // per-task sizes
     auto sizes = [](const Task &T, int64_t &nd, int64_t &nsct, int64_t &nax, int64_t &ny, int64_t &nz) {
         int L = std::max(T.Ls, T.Lt);
-        nd = wig_size(L, T.K);
+        nd = fold_size(L, T.K);
         nsct = sect_size(T.K, T.Lt + T.Ls + 2);
         nax = ax_size(T.Lt, T.Ls, T.K);
         ny = 2 * rows_size(T.Ls, T.K);
@@ -159,13 +164,13 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         cur.n_y += ny;
         cur.n_z += nz;
         int L = std::max(T.Ls, T.Lt);
-        cur.maxpairs = std::max(cur.maxpairs, (2 * T.K + 1) * (2 * L + 1));
+        cur.maxpairs = std::max(cur.maxpairs, (T.K + 1) * (L + 1));
         cur.maxK = std::max(cur.maxK, T.K);
         cur.max_nmax = std::max(cur.max_nmax, T.Lt + T.Ls + 2);
         cur.max_Ls = std::max(cur.max_Ls, T.Ls);
-        cur.max_rows_y = std::max(cur.max_rows_y, rows_size(T.Ls, T.K));
+        cur.max_rows_y = std::max(cur.max_rows_y, frow_size(T.Ls, T.K));
         cur.max_rows_z = std::max(cur.max_rows_z, rows_size(T.Lt, T.K));
-        int nent = T.Lt * (T.Lt + 2);
+        int nent = T.Lt * (T.Lt + 3) / 2;  // back-rotation entries (n, m >= 0)
         cur.rowback_blocks = std::max(cur.rowback_blocks, std::min(64, (nent + 255) / 256));
         double f1, f2, f3;
         task_flops(T, f1, f2, f3);
@@ -238,7 +243,7 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
         if (!(B.cached && B.tables_ready)) {
             {
                 Scope s(c, B.pn[0]);
-                launch_gen_wigner(tt, nt, ch.maxpairs, wd, c->stream);
+                launch_gen_wigner_fold(tt, nt, ch.maxpairs, wd, c->stream);
             }
             {
                 Scope s(c, B.pn[1]);
@@ -346,6 +351,52 @@ int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
 }
 
 // Chunks, runs, segments and reduction lists for nrhs right-hand sides per call.
+// Gauss-Legendre nodes and weights on [-1, 1] by Newton's iteration on P_Q in long
+// double (Golub-Welsch would do as well; Q <= a few thousand here).
+static void gauss_legendre(int Q, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(Q, 0.0);
+    w.assign(Q, 0.0);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int i = 0; i < (Q + 1) / 2; ++i) {
+        long double z = cosl(pi * (i + 0.75L) / (Q + 0.5L)), dp = 1.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int k = 2; k <= Q; ++k) {
+                long double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (Q == 1) p0 = 1.0L;
+            dp = Q * (z * p1 - p0) / (z * z - 1.0L);
+            long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        long double wt = 2.0L / ((1.0L - z * z) * dp * dp);
+        x[i] = (double)-z;
+        x[Q - 1 - i] = (double)z;
+        w[i] = w[Q - 1 - i] = (double)wt;
+    }
+}
+
+int gq_rule(ifs_ctx *c, int Q, const double **mu, const double **w) {
+    auto it = c->gq_nodes.find(Q);
+    if (it == c->gq_nodes.end()) {
+        std::vector<double> x, wt;
+        gauss_legendre(Q, x, wt);
+        DevBuf &d = c->gq_nodes[Q];
+        int rc = d.alloc(16 * (size_t)Q);
+        if (rc) return rc;
+        if (hipMemcpy(d.p, x.data(), 8 * (size_t)Q, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d.as<double>() + Q, wt.data(), 8 * (size_t)Q, hipMemcpyHostToDevice) != hipSuccess)
+            return IFS_E_HIP;
+        it = c->gq_nodes.find(Q);
+    }
+    *mu = it->second.as<double>();
+    *w = it->second.as<double>() + Q;
+    return IFS_OK;
+}
+
 int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     B.chunks.clear();
     B.descs.clear();
@@ -448,10 +499,86 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     B.n_d = fold_size(L, T0.K);
     B.n_s = sect_size(T0.K, T0.Lt + T0.Ls + 2);
     B.n_ax = ax_size(T0.Lt, T0.Ls, T0.K);
+    // table keys (Ls, Lt, K are common to the batch)
+    {
+        std::map<double, int> dk;
+        std::map<std::pair<double, int>, int> ak;
+        B.dkey.assign(B.gen.size(), 0);
+        B.axkey.assign(B.gen.size(), 0);
+        B.dkeys.clear();
+        B.axkey_cls.clear();
+        for (size_t g = 0; g < B.gen.size(); ++g) {
+            const Task &T = B.gen[g];
+            const double a = std::fabs(T.cth);
+            auto it = dk.find(a);
+            if (it == dk.end()) {
+                it = dk.emplace(a, (int)B.dkeys.size()).first;
+                Task K = T;
+                K.cth = a;
+                B.dkeys.push_back(K);
+            }
+            B.dkey[g] = it->second;
+            auto jt = ak.find({T.t, T.kind});
+            if (jt == ak.end()) {
+                jt = ak.emplace(std::make_pair(T.t, T.kind), (int)B.axkey_cls.size()).first;
+                B.axkey_cls.push_back((int)g);
+            }
+            B.axkey[g] = jt->second;
+        }
+    }
+    // coaxial tables by quadrature (stable at every order)
+    B.gq.clear();
+    size_t gq_ws = 0;
+    for (int g : B.axkey_cls) {
+        const Task &T = B.gen[g];
+        B.gq.push_back(gq_plan(T.kind, T.t, T0.Lt, T0.Ls, T0.K, (size_t)1 << 30));
+        gq_ws = std::max(gq_ws, B.gq.back().ws_bytes);
+        const double *mu, *w;
+        int r = gq_rule(c, B.gq.back().Q, &mu, &w);
+        if (r) return r;
+    }
     int rc;
-    if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, B.n_d * 8))) ||
-        (rc = c->ws_s.alloc(std::max<size_t>(c->ws_s.bytes, B.n_s * 16))) ||
-        (rc = c->ws_ax.alloc(std::max<size_t>(c->ws_ax.bytes, B.n_ax * 16))) ||
+    // persistent tables when the cache budget holds all of a kind (Wigner first)
+    if (!B.d_persist) {
+        const size_t need = (size_t)B.n_d * 8 * B.dkeys.size();
+        if (c->ws_cached + need <= c->cache_budget && need > 0) {
+            if ((rc = B.own_d.alloc(need)) != IFS_OK) return rc;
+            c->ws_cached += need;
+            B.d_persist = true;
+            B.d_ready.assign(B.dkeys.size(), 0);
+        }
+    }
+    if (!B.ax_persist) {
+        const size_t need = (size_t)B.n_ax * 16 * B.axkey_cls.size();
+        if (c->ws_cached + need <= c->cache_budget && need > 0) {
+            if ((rc = B.own_ax.alloc(need)) != IFS_OK) return rc;
+            c->ws_cached += need;
+            B.ax_persist = true;
+            B.ax_ready.assign(B.axkey_cls.size(), 0);
+        }
+    }
+    // otherwise Wigner table slots: the tables of a chunk's runs stay for the back pass
+    // when a quarter of the workspace budget holds them (run r uses slot r % nslots)
+    size_t maxruns = 1;
+    for (auto &ch : B.chunks) maxruns = std::max(maxruns, ch.runs.size());
+    const size_t dbytes = (size_t)B.n_d * 8;
+    B.n_dslots = B.d_persist ? 0
+                             : (int)std::max<size_t>(1, std::min(maxruns, (c->ws_budget / 4) / std::max<size_t>(dbytes, 1)));
+    for (auto &ch : B.chunks) {
+        for (size_t r = 0; r < ch.runs.size(); ++r) {
+            auto &P = ch.runs[r];
+            P.g.off_d = B.d_persist ? (int64_t)B.dkey[P.gen] * B.n_d : (int64_t)(r % B.n_dslots) * B.n_d;
+            P.g.mirror = B.gen[P.gen].cth < 0.0 ? 1 : 0;
+            B.descs[P.di] = P.g;
+        }
+        for (auto &P : ch.segs) {
+            P.g.off_ax = B.ax_persist ? (int64_t)B.axkey[P.gen] * B.n_ax : 0;
+            B.descs[P.di] = P.g;
+        }
+    }
+    if ((rc = c->ws_d.alloc(std::max<size_t>(c->ws_d.bytes, B.n_d * 8 * (size_t)B.n_dslots))) ||
+        (rc = c->ws_gq.alloc(std::max<size_t>(c->ws_gq.bytes, gq_ws))) ||
+        (rc = c->ws_ax.alloc(std::max<size_t>(c->ws_ax.bytes, B.ax_persist ? 16 : B.n_ax * 16))) ||
         (rc = c->ws_y.alloc(std::max<size_t>(c->ws_y.bytes, B.n_y * 8))) ||
         (rc = c->ws_z.alloc(std::max<size_t>(c->ws_z.bytes, B.n_z * 8))))
         return rc;
@@ -463,6 +590,7 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     };
     if ((rc = up(B.d_desc, B.descs.data(), sizeof(GroupDesc) * B.descs.size())) ||
         (rc = up(B.d_gen, B.gen.data(), sizeof(Task) * B.gen.size())) ||
+        (rc = up(B.d_dkeys, B.dkeys.data(), sizeof(Task) * B.dkeys.size())) ||
         (rc = up(B.d_src, B.src.data(), 8 * B.src.size())) || (rc = up(B.d_pidx, B.pidx.data(), 4 * B.pidx.size())) ||
         (rc = up(B.d_phase, B.phase.data(), sizeof(double2) * B.phase.size())) ||
         (rc = up(B.d_scat, B.scat.data(), sizeof(ScatterSeg) * B.scat.size())) ||
@@ -480,12 +608,13 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
     }
     const GroupDesc *dd = B.d_desc.as<GroupDesc>();
     const Task *dg = B.d_gen.as<Task>();
+    const Task *dk = B.d_dkeys.as<Task>();
     const int64_t *dsrc = B.d_src.as<int64_t>();
     const int32_t *dpidx = B.d_pidx.as<int32_t>();
     const double2 *dph = B.d_phase.as<double2>();
-    double *wd = c->ws_d.as<double>();
+    double *wd = B.d_persist ? B.own_d.as<double>() : c->ws_d.as<double>();
     double2 *wsc = c->ws_s.as<double2>();
-    double2 *wax = c->ws_ax.as<double2>();
+    double2 *wax = B.ax_persist ? B.own_ax.as<double2>() : c->ws_ax.as<double2>();
     double *wy = c->ws_y.as<double>();
     double *wz = c->ws_z.as<double>();
     const Task &T0 = B.gen[0];
@@ -493,32 +622,50 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
     class_col_flops(T0, ffwd, fax, fback);
     const int L = std::max(T0.Ls, T0.Lt);
     const int64_t nbox = 2LL * T0.Lt * (T0.Lt + 2);
-    int cur_d = -1, cur_ax = -1;  // class whose table is in the workspace
-    auto need_d = [&](int g) {
-        if (cur_d >= 0 && same_d(B.gen[cur_d], B.gen[g])) return;
+    int cur_ax = -1;                                        // coaxial key in the shared workspace
+    std::vector<int> owner(std::max(B.n_dslots, 1), -1);    // Wigner key in each shared slot
+    auto need_d = [&](const ClassBatch::Piece &P) {
+        const int key = B.dkey[P.gen];
+        if (B.d_persist) {
+            if (B.d_ready[key]) return;
+            B.d_ready[key] = 1;
+        } else {
+            int slot = (int)(P.g.off_d / std::max<int64_t>(B.n_d, 1));
+            if (owner[slot] == key) return;
+            owner[slot] = key;
+        }
         Scope s(c, "class_gen_wigner");
-        launch_gen_wigner_fold(dg + g, 1, (T0.K + 1) * (L + 1), wd, c->stream);
-        cur_d = g;
+        launch_gen_wigner_fold(dk + key, 1, (T0.K + 1) * (L + 1), wd + P.g.off_d, c->stream);
+    };
+    auto need_ax = [&](const ClassBatch::Piece &P) {
+        const int key = B.axkey[P.gen];
+        if (B.ax_persist) {
+            if (B.ax_ready[key]) return;
+            B.ax_ready[key] = 1;
+        } else {
+            if (cur_ax == key) return;
+            cur_ax = key;
+        }
+        Scope s(c, "class_gen_axial");
+        const GqPlan &G = B.gq[key];
+        const double *mu = nullptr, *w = nullptr;
+        (void)gq_rule(c, G.Q, &mu, &w);  // created at plan time
+        launch_gen_axial_gq(G, mu, w, c->ws_gq.p, wax + P.g.off_ax, c->stream);
     };
     for (auto &ch : B.chunks) {
         for (auto &P : ch.runs) {
-            need_d(P.gen);
+            need_d(P);
             Scope s(c, "class_rot_fwd", ffwd * P.g.ncols * nrhs);
             launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, c->stream);
         }
         for (auto &P : ch.segs) {
-            if (cur_ax < 0 || !same_ax(B.gen[cur_ax], B.gen[P.gen])) {
-                Scope s(c, "class_gen_axial");
-                launch_gen_sect(dg + P.gen, 1, T0.Lt + T0.Ls + 2, wsc, c->stream);
-                launch_gen_axial(dg + P.gen, 1, T0.K, T0.Lt + T0.Ls + 2, wsc, wax, c->stream);
-                cur_ax = P.gen;
-            }
+            need_ax(P);
             Scope s(c, "class_axial", fax * P.g.ncols * nrhs);
             launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, c->stream);
         }
         for (size_t r = ch.runs.size(); r-- > 0;) {
             auto &P = ch.runs[r];
-            need_d(P.gen);
+            need_d(P);
             Scope s(c, "class_rot_back", fback * P.g.ncols * nrhs);
             launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, c->stream);
         }
@@ -672,6 +819,8 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     c->ws_budget = std::min<size_t>(freeb / 4, (size_t)48 << 30);
     if (p->workspace_mb > 0) c->ws_budget = (size_t)p->workspace_mb << 20;
     c->cache_budget = freeb / 5 * 2;
+    if (p->cache_mb > 0) c->cache_budget = (size_t)p->cache_mb << 20;
+    if (p->cache_mb < 0) c->cache_budget = 0;
     if (p->d_convention != IFS_D_PHYSICAL && p->d_convention != IFS_D_REFERENCE) return fail(IFS_E_ARG);
     c->dsign = p->d_convention == IFS_D_REFERENCE ? 1.0 : -1.0;
 

@@ -1,0 +1,384 @@
+// Coaxial translation tables by Gauss-Legendre quadrature (stable at high order).
+//
+// The scalar coaxial coefficients of order m are integrals over mu = cos(theta):
+//   alpha^m(n, nu) = i^{n - nu} * int_{-1}^{1} Pb_n^m(mu) Pb_nu^m(mu) T(mu) dmu
+// with Pb_n^m the orthonormal associated Legendre functions and
+//   T(mu) = exp(i t mu)                                   (regular, j_n)
+//   T(mu) = sum_{p=0}^{P} i^p (2p+1) h1_p(t) P_p(mu)      (outgoing -> regular, P = Lt + 1 + Ls)
+// (for j_p the second sum is exp(i t mu) itself: the plane-wave expansion).  The
+// integrand is a polynomial of degree n + nu + P (or an entire function whose
+// Legendre coefficients (2p+1) j_p(t) vanish to rounding beyond p ~ t + 8 t^(1/3) + 30),
+// so Q Gauss nodes with 2Q > n + nu + P integrate it to rounding.  Unlike the
+// three-term recurrences in nu and m, whose parasitic solutions grow without bound
+// once the orders exceed ~100 near the turning point n ~ t, every term here is
+// bounded.
+//
+// For the outgoing kind, |h1_p(t)| grows steeply for p > t, and a single T would
+// put the rounding of its largest terms (~1e10 at p = 2L) into every entry.  T is
+// therefore split in bands of p, a new band starting wherever |h1_p| exceeds 1e3
+// times its value at the band's first order; band b contributes only to the
+// entries with n + nu >= its first order (below that its integral vanishes
+// exactly), so each entry sees rounding of at most 1e3 times the size of its own
+// largest terms.
+//
+// Pipeline per table: T bands at the nodes (k_gq_T), Pb_n^m at the nodes for a
+// batch of orders m (k_gq_leg), alpha for the batch (k_gq_alpha, a banded
+// V^T diag(T) V product), then the vector coefficients and the P = A + B,
+// Q = A - B blocks in the layout of the recurrence generator (k_gq_vector).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <vector>
+
+#include "ifs_internal.h"
+
+namespace ifs {
+
+namespace {
+
+__device__ __forceinline__ double2 cmulg(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// i^k * v
+__device__ __forceinline__ double2 ipowg(int k, double2 v) {
+    switch (k & 3) {
+        case 0: return v;
+        case 1: return make_double2(-v.y, v.x);
+        case 2: return make_double2(-v.x, -v.y);
+        default: return make_double2(v.y, -v.x);
+    }
+}
+__device__ __forceinline__ double coef_ag(int n, int m) {
+    int am = m < 0 ? -m : m;
+    if (n < am) return 0.0;
+    return sqrt((double)(n + 1 + am) * (double)(n + 1 - am) / ((double)(2 * n + 1) * (double)(2 * n + 3)));
+}
+
+// h1_p(t) = j_p(t) + i y_p(t), p = 0..P, one thread: j by Miller's downward
+// recurrence, y upward.
+__device__ void hankel_into(double x, int P, double2 *out) {
+    const double big = (double)P > x ? (double)P : x;
+    const int nstart = (int)big + 30 + (int)sqrt(40.0 * big);
+    double fp1 = 0.0, f = 1e-200;
+    for (int n = nstart; n >= 1; --n) {
+        double fm1 = (2.0 * n + 1.0) / x * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (n - 1 <= P) out[n - 1].x = f;
+        if (fabs(f) > 1e200) {
+            f *= 1e-200;
+            fp1 *= 1e-200;
+            if (n - 1 <= P)
+                for (int q = n - 1; q <= P; ++q) out[q].x *= 1e-200;
+        }
+    }
+    const double j0 = sin(x) / x;
+    const double j1 = sin(x) / (x * x) - cos(x) / x;
+    const double scale = (P >= 1 && fabs(j1) > fabs(j0)) ? j1 / out[1].x : j0 / out[0].x;
+    for (int n = 0; n <= P; ++n) out[n].x *= scale;
+    double ym1 = -cos(x) / x;
+    out[0].y = ym1;
+    if (P >= 1) {
+        double y = -cos(x) / (x * x) - sin(x) / x;
+        out[1].y = y;
+        for (int n = 1; n < P; ++n) {
+            double yp = (2.0 * n + 1.0) / x * y - ym1;
+            ym1 = y;
+            y = yp;
+            out[n + 1].y = y;
+        }
+    }
+}
+
+}  // namespace
+
+// T bands at the nodes, weights folded in: Tb[b * Q + q] = w_q T_b(mu_q).
+//   kind 1: one band, exp(i t mu);  kind 3: band b = terms p in [lo[b], lo[b+1]).
+__global__ void k_gq_T(const GqPlan g, const double *__restrict__ mu, const double *__restrict__ w,
+                       double2 *__restrict__ Tb) {
+    extern __shared__ double2 hsh[];
+    const int Q = g.Q;
+    if (g.kind == 3) {
+        if (threadIdx.x == 0) hankel_into(g.t, g.P, hsh);
+        __syncthreads();
+    }
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const double x = mu[q], wq = w[q];
+    if (g.kind != 3) {
+        double s, c;
+        sincos(g.t * x, &s, &c);
+        Tb[q] = make_double2(wq * c, wq * s);
+        return;
+    }
+    double2 acc = make_double2(0.0, 0.0);
+    int band = 0;
+    double pm1 = 0.0, pc = 1.0;  // P_{p-1}, P_p
+    for (int p = 0; p <= g.P; ++p) {
+        if (p >= 1) {
+            double pn = (p == 1) ? x : ((2.0 * p - 1.0) * x * pc - (p - 1.0) * pm1) / p;
+            pm1 = pc;
+            pc = pn;
+        }
+        while (band + 1 < g.nb && p >= g.lo[band + 1]) {
+            Tb[(int64_t)band * Q + q] = make_double2(wq * acc.x, wq * acc.y);
+            acc = make_double2(0.0, 0.0);
+            ++band;
+        }
+        double2 cp = ipowg(p, hsh[p]);
+        const double f = (2.0 * p + 1.0) * pc;
+        acc.x += f * cp.x;
+        acc.y += f * cp.y;
+    }
+    Tb[(int64_t)band * Q + q] = make_double2(wq * acc.x, wq * acc.y);
+    for (int b = band + 1; b < g.nb; ++b) Tb[(int64_t)b * Q + q] = make_double2(0.0, 0.0);
+}
+
+// V[(j * (Nr + 1) + n) * Q + q] = Pb_n^{m0 + j}(mu_q), n = 0..Nr (0 for n < m).
+__global__ void k_gq_leg(int m0, int nm, int Nr, int Q, const double *__restrict__ mu, double *__restrict__ V) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (q >= Q || j >= nm) return;
+    const int m = m0 + j;
+    double *out = V + (int64_t)j * (Nr + 1) * Q + q;
+    for (int n = 0; n < m && n <= Nr; ++n) out[(int64_t)n * Q] = 0.0;
+    if (m > Nr) return;
+    const double x = mu[q];
+    const double s2 = fmax(1.0 - x * x, 0.0);
+    // Pb_m^m = sqrt((2m+1)/2 (2m)!/(2^m m!)^2) (1 - x^2)^(m/2), in logs (underflow -> 0)
+    const double lc = 0.5 * (log((2.0 * m + 1.0) * 0.5) + lgamma(2.0 * m + 1.0) -
+                             2.0 * (m * 0.6931471805599453 + lgamma(m + 1.0)));
+    double pmm = (m == 0) ? exp(lc) : (s2 > 0.0 ? exp(lc + 0.5 * m * log(s2)) : 0.0);
+    out[(int64_t)m * Q] = pmm;
+    if (m + 1 > Nr) return;
+    double p1 = sqrt(2.0 * m + 3.0) * x * pmm;
+    out[(int64_t)(m + 1) * Q] = p1;
+    double p2 = pmm;
+    const double mm = (double)m * m;
+    for (int n = m + 2; n <= Nr; ++n) {
+        const double nn = (double)n;
+        const double a = sqrt((4.0 * nn * nn - 1.0) / (nn * nn - mm));
+        const double b = sqrt(((nn - 1.0) * (nn - 1.0) - mm) * (2.0 * nn + 1.0) / ((nn * nn - mm) * (2.0 * nn - 3.0)));
+        const double pn = a * x * p1 - b * p2;
+        out[(int64_t)n * Q] = pn;
+        p2 = p1;
+        p1 = pn;
+    }
+}
+
+// alpha[(j * (Lt + 2) + n) * (Ls + 1) + nu] = i^{n-nu} sum_b [n + nu >= lo_b] sum_q V_n V_nu Tb_b,
+// n = 0..Lt+1, nu = 0..Ls.  Block: 64 x 64 entries, thread: 4 x 4, inner step 16 nodes.
+constexpr int GT = 64;
+constexpr int GK = 16;
+__global__ __launch_bounds__(256) void k_gq_alpha(const GqPlan g, int nm, const double *__restrict__ V,
+                                                  const double2 *__restrict__ Tb, double2 *__restrict__ alpha) {
+    const int Lt = g.Lt, Ls = g.Ls, Nr = g.Nr, Q = g.Q;
+    const int j = blockIdx.z;
+    const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
+    const int NR = Lt + 2, NCc = Ls + 1;
+    if (r0 >= NR || c0 >= NCc) return;
+    __shared__ double Vr[GK][GT + 1];
+    __shared__ double Vc[GK][GT + 1];
+    __shared__ double2 Ts[GK];
+    const double *Vm = V + (int64_t)j * (Nr + 1) * Q;
+    const int tid = threadIdx.x;
+    const int tr = (tid >> 4) * 4, tc = (tid & 15) * 4;
+    double ar[4][4], ai[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) ar[a][b] = ai[a][b] = 0.0;
+    const int rmax = min(NR - 1, r0 + GT - 1), cmax = min(NCc - 1, c0 + GT - 1);
+    for (int band = 0; band < g.nb; ++band) {
+        const int lo = g.lo[band];
+        if (lo > rmax + cmax) break;
+        const double2 *T = Tb + (int64_t)band * Q;
+        double br[4][4], bi[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) br[a][b] = bi[a][b] = 0.0;
+        for (int k0 = 0; k0 < Q; k0 += GK) {
+            // stage 16 nodes x 64 rows and 64 columns (coalesced along q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int idx = tid + 256 * e;  // 64 x 16
+                int rr = idx >> 4, qq = idx & 15;
+                int gq = k0 + qq;
+                int gr = r0 + rr, gc = c0 + rr;
+                Vr[qq][rr] = (gq < Q && gr <= Nr && gr < NR) ? Vm[(int64_t)gr * Q + gq] : 0.0;
+                Vc[qq][rr] = (gq < Q && gc <= Nr && gc < NCc) ? Vm[(int64_t)gc * Q + gq] : 0.0;
+            }
+            if (tid < GK) Ts[tid] = (k0 + tid < Q) ? T[k0 + tid] : make_double2(0.0, 0.0);
+            __syncthreads();
+#pragma unroll 4
+            for (int qq = 0; qq < GK; ++qq) {
+                const double2 tq = Ts[qq];
+                double vr[4], vc[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    vr[a] = Vr[qq][tr + a] * tq.x;
+                    vc[a] = Vc[qq][tc + a];
+                }
+                double vi[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) vi[a] = Vr[qq][tr + a] * tq.y;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        br[a][b] = fma(vr[a], vc[b], br[a][b]);
+                        bi[a][b] = fma(vi[a], vc[b], bi[a][b]);
+                    }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int n = r0 + tr + a, nu = c0 + tc + b;
+                if (n + nu >= lo) {
+                    ar[a][b] += br[a][b];
+                    ai[a][b] += bi[a][b];
+                }
+            }
+    }
+    double2 *out = alpha + (int64_t)j * NR * NCc;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int n = r0 + tr + a, nu = c0 + tc + b;
+            if (n < NR && nu < NCc) out[(int64_t)n * NCc + nu] = ipowg(((n - nu) % 4 + 4) & 3, make_double2(ar[a][b], ai[a][b]));
+        }
+}
+
+// Vector coaxial coefficients (normalised basis) from alpha, orders m0..m0+nm-1:
+//   A_m(n,nu) = i^{nu-n} [sqrt(n(n+1)) al(n,nu) + t a_n sqrt(n/(n+1)) al(n+1,nu)
+//                         + t a_{n-1} sqrt((n+1)/n) al(n-1,nu)] / sqrt(nu(nu+1))
+//   B_m(n,nu) = i^{nu-n+1} t m al(n,nu) / sqrt(n(n+1) nu(nu+1))
+// stored as P = A + B, Q = A - B (order -m: A unchanged, B negated), in the block
+// layout of the recurrence generator.
+__global__ void k_gq_vector(int m0, int nm, int Lt, int Ls, int K, double t, const double2 *__restrict__ alpha,
+                            double2 *__restrict__ wax) {
+    const int j = blockIdx.y;
+    const int m = m0 + j;
+    if (j >= nm || m > K) return;
+    const int q = m < 1 ? 1 : m;
+    const int nrow = Lt - q + 1, ncol = Ls - q + 1;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nrow <= 0 || ncol <= 0 || e >= (int64_t)nrow * ncol) return;
+    const int n = q + (int)(e / ncol), nu = q + (int)(e % ncol);
+    const int NR = Lt + 2, NCc = Ls + 1;
+    const double2 *al = alpha + (int64_t)j * NR * NCc;
+    const double2 c0 = al[(int64_t)n * NCc + nu];
+    const double2 cp = al[(int64_t)(n + 1) * NCc + nu];
+    const double2 cm = (n - 1 >= m) ? al[(int64_t)(n - 1) * NCc + nu] : make_double2(0.0, 0.0);
+    const double nn = (double)n;
+    const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1.0));
+    const double f0 = sqrt(nn * (nn + 1.0)), fp = t * coef_ag(n, m) * sqrt(nn / (nn + 1.0)),
+                 fm = t * coef_ag(n - 1, m) * sqrt((nn + 1.0) / nn);
+    double2 s = make_double2(f0 * c0.x + fp * cp.x + fm * cm.x, f0 * c0.y + fp * cp.y + fm * cm.y);
+    const double2 av = ipowg(((nu - n) % 4 + 4) & 3, make_double2(inv_nu * s.x, inv_nu * s.y));
+    const double fb = t * (double)m * inv_nu / sqrt(nn * (nn + 1.0));
+    const double2 bv = ipowg(((nu - n + 1) % 4 + 4) & 3, make_double2(fb * c0.x, fb * c0.y));
+    const double2 pv = make_double2(av.x + bv.x, av.y + bv.y), qv = make_double2(av.x - bv.x, av.y - bv.y);
+    double2 *Ap = wax + ax_off(m, Lt, Ls, K);
+    double2 *Bp = Ap + ax_block(m, Lt, Ls);
+    const int64_t o = (int64_t)(n - q) * ncol + (nu - q);
+    Ap[o] = pv;
+    Bp[o] = qv;
+    if (m > 0) {
+        double2 *An = wax + ax_off(-m, Lt, Ls, K);
+        double2 *Bn = An + ax_block(-m, Lt, Ls);
+        An[o] = qv;
+        Bn[o] = pv;
+    }
+}
+
+// |h1_p(t)|, p = 0..P, on the host (Miller's recurrence for j_p, upward for y_p).
+static void hankel_abs_host(double x, int P, std::vector<double> &out) {
+    std::vector<double> j(P + 1, 0.0), y(P + 1, 0.0);
+    const double big = (double)P > x ? (double)P : x;
+    const int nstart = (int)big + 30 + (int)sqrt(40.0 * big);
+    double fp1 = 0.0, f = 1e-200;
+    for (int n = nstart; n >= 1; --n) {
+        double fm1 = (2.0 * n + 1.0) / x * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (n - 1 <= P) j[n - 1] = f;
+        if (fabs(f) > 1e200) {
+            f *= 1e-200;
+            fp1 *= 1e-200;
+            if (n - 1 <= P)
+                for (int q = n - 1; q <= P; ++q) j[q] *= 1e-200;
+        }
+    }
+    const double j0 = sin(x) / x, j1 = sin(x) / (x * x) - cos(x) / x;
+    const double scale = (P >= 1 && fabs(j1) > fabs(j0)) ? j1 / j[1] : j0 / j[0];
+    y[0] = -cos(x) / x;
+    if (P >= 1) y[1] = -cos(x) / (x * x) - sin(x) / x;
+    for (int n = 1; n < P; ++n) y[n + 1] = (2.0 * n + 1.0) / x * y[n] - y[n - 1];
+    out.resize(P + 1);
+    for (int n = 0; n <= P; ++n) out[n] = hypot(j[n] * scale, y[n]);
+}
+
+// Plan of one table: node count, band split, workspace.
+GqPlan gq_plan(int kind, double t, int Lt, int Ls, int K, size_t ws_cap_bytes) {
+    GqPlan g;
+    g.kind = kind;
+    g.t = t;
+    g.Lt = Lt;
+    g.Ls = Ls;
+    g.K = K;
+    int Pdeg;
+    g.nb = 1;
+    g.lo[0] = 0;
+    if (kind == 3) {
+        g.P = Lt + 1 + Ls;
+        Pdeg = g.P;
+        std::vector<double> h;
+        hankel_abs_host(t, g.P, h);
+        double ref = h[0];
+        for (int p = 1; p <= g.P; ++p) {
+            if (h[p] > 1e3 * ref && g.nb < GQ_MAXB) {
+                g.lo[g.nb++] = p;
+                ref = h[p];
+            }
+        }
+    } else {
+        g.P = 0;
+        Pdeg = (int)ceil(t + 8.0 * cbrt(t) + 30.0);
+    }
+    g.Q = (Lt + 1 + Ls + Pdeg) / 2 + 4;
+    g.Nr = (Lt + 1 > Ls ? Lt + 1 : Ls);
+    const size_t per_m = (size_t)(g.Nr + 1) * g.Q * 8 + (size_t)(Lt + 2) * (Ls + 1) * 16;
+    int mb = (int)(ws_cap_bytes / (per_m ? per_m : 1));
+    if (mb < 1) mb = 1;
+    if (mb > K + 1) mb = K + 1;
+    g.mb = mb;
+    g.ws_bytes = per_m * mb + (size_t)g.nb * g.Q * 16;
+    return g;
+}
+
+void launch_gen_axial_gq(const GqPlan &g, const double *mu, const double *w, void *ws, double2 *wax, hipStream_t s) {
+    double2 *Tb = (double2 *)ws;
+    double *V = (double *)(Tb + (size_t)g.nb * g.Q);
+    double2 *alpha = (double2 *)(V + (size_t)g.mb * (g.Nr + 1) * g.Q);
+    {
+        size_t shm = g.kind == 3 ? sizeof(double2) * (size_t)(g.P + 1) : 0;
+        hipLaunchKernelGGL(k_gq_T, dim3((g.Q + 255) / 256), dim3(256), shm, s, g, mu, w, Tb);
+    }
+    for (int m0 = 0; m0 <= g.K; m0 += g.mb) {
+        const int nm = (g.K + 1 - m0) < g.mb ? (g.K + 1 - m0) : g.mb;
+        hipLaunchKernelGGL(k_gq_leg, dim3((g.Q + 255) / 256, nm), dim3(256), 0, s, m0, nm, g.Nr, g.Q, mu, V);
+        dim3 ga((g.Ls + 1 + GT - 1) / GT, (g.Lt + 2 + GT - 1) / GT, nm);
+        hipLaunchKernelGGL(k_gq_alpha, ga, dim3(256), 0, s, g, nm, V, Tb, alpha);
+        const int64_t ent = (int64_t)g.Lt * g.Ls;
+        hipLaunchKernelGGL(k_gq_vector, dim3((unsigned)((ent + 255) / 256), nm), dim3(256), 0, s, m0, nm, g.Lt, g.Ls,
+                           g.K, g.t, alpha, wax);
+    }
+}
+
+}  // namespace ifs

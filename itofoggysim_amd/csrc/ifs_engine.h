@@ -131,6 +131,8 @@ struct ifs_ctx {
     ifs::DevBuf d_Leff, d_active, d_off, d_snm, d_an, d_bn, d_x, d_b;
     ifs::DevBuf d_xh, d_acc;             // compact normalised input / accumulator
     ifs::DevBuf ws_d, ws_s, ws_ax, ws_y, ws_z;
+    ifs::DevBuf ws_gq;                       // quadrature generator workspace
+    std::map<int, ifs::DevBuf> gq_nodes;     // Q -> Gauss-Legendre nodes then weights (2Q doubles)
     double dsign = -1.0;            // d = dsign * (origin(source) - origin(target))
     size_t ws_budget = 0;           // translation workspace (chunked batches)
     size_t cache_budget = 0;        // tables kept across matvecs
@@ -179,6 +181,17 @@ struct ClassBatch {
     std::vector<int32_t> scat_cols;
     int PL = 0;
     int plan_nrhs = 0;                 // right-hand sides the plan was cut for
+    int n_dslots = 1;                  // Wigner tables kept at once (chunk runs share slots r % n_dslots)
+    // Table keys: one Wigner table per |cos(theta)| (a run of cos(theta) < 0 mirrors
+    // it), one coaxial table per (length, kind).  When the context's cache budget
+    // holds every key's table, the tables persist across matvecs (generated once).
+    std::vector<int> dkey, axkey;      // per class
+    std::vector<Task> dkeys;           // per Wigner key: generator task (cth = |cth|)
+    std::vector<int> axkey_cls;        // per coaxial key: a class that generates it
+    std::vector<GqPlan> gq;            // per coaxial key: quadrature plan
+    std::vector<char> d_ready, ax_ready;
+    bool d_persist = false, ax_persist = false;
+    DevBuf own_d, own_ax, d_dkeys;
     int64_t src_stride = 0, dst_stride = 0;  // pool stride between right-hand sides (complex)
     DevBuf d_desc, d_gen, d_src, d_pidx, d_phase, d_scat, d_scat_cols;
     int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;  // workspace (doubles for d, y, z; complex for s, ax)
@@ -188,6 +201,8 @@ struct ClassBatch {
 // != 0 means x and b are in the normalised basis.
 int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb = -1);
 int class_batch_finalize(ifs_ctx *c, ClassBatch &B);
+// Device Gauss-Legendre rule of Q nodes (cached per context): mu, then w.
+int gq_rule(ifs_ctx *c, int Q, const double **mu, const double **w);
 int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs);
 int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs);
 void class_batch_cost(const ClassBatch &B, double &flops, double &bytes);

@@ -48,10 +48,16 @@ constexpr int RPAD = 2;  // LDS row padding (doubles)
 //   back, degree n:      F = E^T s,  G = O^T t,  s_k = z_k + (-1)^k z_{-k} (s_0 = z_0),
 //                        t_k = z_k - (-1)^k z_{-k}
 //                        w_m = (F_m + G_m)/2,  w_{-m} = (-1)^m (F_m - G_m)/2
-// which is half the work of the unfolded products.  Block: 32 rows of each
-// product x 128 real columns; waves 0,1 compute the E product, waves 2,3 the O
-// product (64 columns each), and the halves meet through LDS in the epilogue.
-constexpr int FM = 32;
+// which is half the work of the unfolded products.  A run of polar angle pi - b
+// (mirror) uses the tables of b: with sigma = (-1)^{n+k+m},
+//   E(pi - b) = sigma E(b),  O(pi - b) = -sigma O(b),
+// so the column entries take (-1)^m (forward) or (-1)^k (back) at the LDS store and
+// the results (-1)^{n+k} (forward) or (-1)^{n+m} (back), with T and G negated.
+// Block: 64 rows of each
+// product x 128 real columns; wave w owns rows 32 (w / 2) .. + 31 and columns
+// 64 (w % 2) .. + 63 of both products, so S and T (F and G) of one entry meet in
+// the same lane.  Each gathered column element serves 64 rows of each table.
+constexpr int FM = 64;
 constexpr int FN = 128;
 constexpr int FK = 16;
 
@@ -74,20 +80,23 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     const int row0 = blockIdx.y * FM;
     const int col0 = blockIdx.x * FN;
     if (row0 >= M) return;
-    __shared__ double As[FK][2 * FM + RPAD];     // columns 0..31: E rows, 32..63: O rows
+    __shared__ double As[FK][2 * FM + RPAD];     // columns 0..FM-1: E rows, FM..2FM-1: O rows
     __shared__ double Bs[2][FK][FN + RPAD];      // s, t
     const double *E = wd + C.off_d + fold_off(nu, K);
     const double *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int g = w >> 1;            // 0: E product, 1: O product
-    const int wc = (w & 1) * 64;
-    dv4 acc[2][4];
+    const int wr = (w >> 1) * 32;    // wave rows
+    const int wc = (w & 1) * 64;     // wave columns
+    dv4 accS[2][4], accT[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j) {
+            accS[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+            accT[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        }
     const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
     // gather: thread -> (translation t, inner q), idx = tid + 256 e, t = idx / 16, q = idx % 16
     const double2 *xs[2];
@@ -101,15 +110,15 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         xs[e] = ok ? src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0) : nullptr;  // xs[m]
         ps[e] = ok ? phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL : nullptr;
     }
-    double ra[4];
+    double ra[8];
     double2 rap[2], rbp[2], ram[2], rbm[2], rph[2];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 64 rows (E 32, O 32) x 16 inner
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;  // 2 FM rows (E FM, O FM) x 16 inner
             int r = idx >> 4, q = idx & 15;
-            int gr = row0 + (r & 31), gq = k0 + q;
-            const double *src = (r < 32) ? E : O;
+            int gr = row0 + (r & (FM - 1)), gq = k0 + q;
+            const double *src = (r < FM) ? E : O;
             ra[e] = (gr < M && gq < KI) ? src[(int64_t)gr * KI + gq] : 0.0;
         }
 #pragma unroll
@@ -130,7 +139,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     load(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < 8; ++e) {
             int idx = tid + 256 * e;
             As[idx & 15][idx >> 4] = ra[e];
         }
@@ -153,6 +162,10 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             if (m == 0) {
                 sg = 0.0;  // s_0 = u_0, t_0 = 0 (O has a zero column 0)
             }
+            if (C.mirror && (m & 1)) {  // (-1)^m on s_m and t_m
+                u0r = -u0r, u0i = -u0i, u1r = -u1r, u1i = -u1i;
+                v0r = -v0r, v0i = -v0i, v1r = -v1r, v1i = -v1i;
+            }
             Bs[0][q][4 * t + 0] = u0r + sg * v0r;
             Bs[0][q][4 * t + 1] = u0i + sg * v0i;
             Bs[0][q][4 * t + 2] = u1r + sg * v1r;
@@ -166,30 +179,27 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         if (k0 + FK < KI) load(k0 + FK);
 #pragma unroll
         for (int kc = 0; kc < FK; kc += 4) {
-            double a[2], b[4];
+            double ae[2], ao[2], bs[4], bt[4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
+            for (int i = 0; i < 2; ++i) {
+                ae[i] = As[kc + lk][wr + 16 * i + li];
+                ao[i] = As[kc + lk][FM + wr + 16 * i + li];
+            }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
+            for (int j = 0; j < 4; ++j) {
+                bs[j] = Bs[0][kc + lk][wc + 16 * j + li];
+                bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
+            }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; ++j) {
+                    accS[i][j] = mfma(ae[i], bs[j], accS[i][j]);
+                    accT[i][j] = mfma(ao[i], bt[j], accT[i][j]);
+                }
         }
         __syncthreads();
     }
-    // epilogue: the O-product waves hand T to the E-product waves through LDS
-    double *X = &Bs[0][0][0];  // 32 x (FN + RPAD) doubles
-    if (g == 1) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FN + RPAD) + wc + 16 * j + li] = acc[i][j][v];
-    }
-    __syncthreads();
-    if (g == 1) return;
     double *Y = wy + rows_off(nu, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -200,10 +210,14 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             if (gc >= NC) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int r = 16 * i + 4 * v + lk;
-                int k = row0 + r;
+                int k = row0 + wr + 16 * i + 4 * v + lk;
                 if (k >= M) continue;
-                double S = acc[i][j][v], T = X[r * (FN + RPAD) + c];
+                double S = accS[i][j][v], T = accT[i][j][v];
+                if (C.mirror) {
+                    double f = ((nu + k) & 1) ? -1.0 : 1.0;
+                    S *= f;
+                    T *= -f;
+                }
                 Y[(int64_t)(kk + k) * NCY + gc] = 0.5 * (S + T);
                 if (k > 0) Y[(int64_t)(kk - k) * NCY + gc] = ((k & 1) ? -0.5 : 0.5) * (S - T);
             }
@@ -228,31 +242,34 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const GroupDesc *__restr
     const int row0 = blockIdx.y * FM;
     const int col0 = blockIdx.x * FN;
     if (row0 >= M) return;
-    __shared__ double As[FK][2 * FM + RPAD];
-    __shared__ double Bs[2][FK][FN + RPAD];
+    __shared__ double As[FK][2 * FM + RPAD];     // columns 0..FM-1: E^T rows, FM..2FM-1: O^T rows
+    __shared__ double Bs[2][FK][FN + RPAD];      // s, t
     const double *E = wd + C.off_d + fold_off(n, K);  // E[k][m], KI x M
     const double *O = E + (int64_t)KI * M;
     const double *Z = wz + rows_off(n, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;  // row kk + k
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int g = w >> 1;
+    const int wr = (w >> 1) * 32;
     const int wc = (w & 1) * 64;
-    dv4 acc[2][4];
+    dv4 accF[2][4], accG[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j) {
+            accF[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+            accG[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        }
     // raw loads only: the s/t combination happens at the LDS store, so that the
     // prefetch of the next step stays in flight while the matrix cores work
-    double ra[4], rzp[8], rzm[8];
+    double ra[8], rzp[8], rzm[8];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 16 inner x 64 rows (E 32, O 32), coalesced along m
-            int q = idx >> 6, r = idx & 63;
-            int gm = row0 + (r & 31), gk = k0 + q;
-            const double *src = (r < 32) ? E : O;
+        for (int e = 0; e < 8; ++e) {
+            int idx = tid + 256 * e;  // 16 inner x 2 FM rows (E FM, O FM), coalesced along m
+            int q = idx >> 7, r = idx & 127;
+            int gm = row0 + (r & (FM - 1)), gk = k0 + q;
+            const double *src = (r < FM) ? E : O;
             ra[e] = (gm < M && gk < KI) ? src[(int64_t)gk * M + gm] : 0.0;
         }
 #pragma unroll
@@ -268,9 +285,9 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const GroupDesc *__restr
     load(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < 8; ++e) {
             int idx = tid + 256 * e;
-            As[idx >> 6][idx & 63] = ra[e];
+            As[idx >> 7][idx & 127] = ra[e];
         }
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -279,36 +296,35 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const GroupDesc *__restr
             int gk = k0 + q;
             // s_k = z_k + (-1)^k z_{-k}, t_k = z_k - (-1)^k z_{-k}; s_0 = z_0, t_0 = 0 (rzm = 0 at k = 0)
             double sg = (gk & 1) ? -1.0 : 1.0;
-            Bs[0][q][idx & 127] = rzp[e] + sg * rzm[e];
-            Bs[1][q][idx & 127] = rzp[e] - sg * rzm[e];
+            double f = (C.mirror && (gk & 1)) ? -1.0 : 1.0;  // (-1)^k on s_k and t_k
+            Bs[0][q][idx & 127] = f * (rzp[e] + sg * rzm[e]);
+            Bs[1][q][idx & 127] = f * (rzp[e] - sg * rzm[e]);
         }
         __syncthreads();
         if (k0 + FK < KI) load(k0 + FK);
 #pragma unroll
         for (int kc = 0; kc < FK; kc += 4) {
-            double a[2], b[4];
+            double ae[2], ao[2], bs[4], bt[4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
+            for (int i = 0; i < 2; ++i) {
+                ae[i] = As[kc + lk][wr + 16 * i + li];
+                ao[i] = As[kc + lk][FM + wr + 16 * i + li];
+            }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
+            for (int j = 0; j < 4; ++j) {
+                bs[j] = Bs[0][kc + lk][wc + 16 * j + li];
+                bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
+            }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; ++j) {
+                    accF[i][j] = mfma(ae[i], bs[j], accF[i][j]);
+                    accG[i][j] = mfma(ao[i], bt[j], accG[i][j]);
+                }
         }
         __syncthreads();
     }
-    double *X = &Bs[0][0][0];
-    if (g == 1) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FN + RPAD) + wc + 16 * j + li] = acc[i][j][v];
-    }
-    __syncthreads();
-    if (g == 1) return;
     // w_m and w_{-m}; the 4 parts of a translation sit in lanes li..li+3 (li % 4 == 0)
     const int64_t ndst = (int64_t)C.Lt * (C.Lt + 2);
 #pragma unroll
@@ -317,14 +333,18 @@ __global__ __launch_bounds__(256) void k_class_rot_back(const GroupDesc *__restr
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int r = 16 * i + 4 * v + lk;
                 int c = wc + 16 * j + li;
-                double F = acc[i][j][v], G = X[r * (FN + RPAD) + c];
+                double F = accF[i][j][v], G = accG[i][j][v];
+                if (C.mirror) {
+                    double f = ((n + row0 + wr + 16 * i + 4 * v + lk) & 1) ? -1.0 : 1.0;  // (-1)^{n+m}
+                    F *= f;
+                    G *= -f;
+                }
                 double wp = 0.5 * (F + G), wm = 0.5 * (F - G);
                 double p1 = __shfl_down(wp, 1, 64), p2 = __shfl_down(wp, 2, 64), p3 = __shfl_down(wp, 3, 64);
                 double q1 = __shfl_down(wm, 1, 64), q2 = __shfl_down(wm, 2, 64), q3 = __shfl_down(wm, 3, 64);
                 int gc = col0 + c;
-                int m = row0 + r;
+                int m = row0 + wr + 16 * i + 4 * v + lk;
                 if ((li & 3) != 0 || gc >= NC || m >= M) continue;
                 int t = gc >> 2;
                 int tt = t % C.ncols, rr = t / C.ncols;  // translation, right-hand side

@@ -56,7 +56,8 @@ struct GroupDesc {
     int32_t ncy;       // translations of the chunk
     int32_t PL;        // phase rows hold exp(i m phi), m = -PL..PL
     int32_t nrhs;      // right-hand sides
-    int32_t pad;
+    int32_t mirror;    // run of the supplementary polar angle pi - b: its tables are those of b,
+                       // with d^n_{km}(pi - b) = (-1)^{n+k} d^n_{k,-m}(b) applied in the products
     int64_t src_stride, dst_stride;
 };
 
@@ -74,16 +75,6 @@ struct Segment {
 };
 
 // Sizes (in doubles for d, in complex entries otherwise).
-__host__ __device__ inline int64_t wig_size(int Lmax, int K) {
-    // sum_{n=1}^{Lmax} (2 min(n,K)+1)(2n+1)
-    int64_t s = 0;
-    int nk = Lmax < K ? Lmax : K;
-    s = (int64_t)(nk + 1) * (2 * nk + 1) * (2 * nk + 3) / 3 - 1;  // n = 1..nk: (2n+1)^2
-    if (Lmax > K) s += (int64_t)(2 * K + 1) * ((int64_t)(Lmax + 1) * (Lmax + 1) - (int64_t)(K + 1) * (K + 1));
-    return s;
-}
-__host__ __device__ inline int64_t wig_off(int n, int K) { return wig_size(n - 1, K); }
-
 // Folded Wigner tables: per degree n (1..L), with kk = min(n,K), the matrices
 //   E[k][m] = d^n_{km} + (-1)^m d^n_{k,-m},  O[k][m] = d^n_{km} - (-1)^m d^n_{k,-m}
 // for k = 0..kk, m = 0..n (so E[k][0] = 2 d^n_{k0}, O[k][0] = O[0][m] = 0), row-major,
@@ -106,6 +97,13 @@ __host__ __device__ inline int64_t rows_size(int L, int K) {
     return s;
 }
 __host__ __device__ inline int64_t rows_off(int n, int K) { return rows_size(n - 1, K); }
+// Rows (n, k >= 0) of degrees 1..L: sum_{n=1}^{L} (min(n,K)+1).
+__host__ __device__ inline int64_t frow_size(int L, int K) {
+    int nk = L < K ? L : K;
+    int64_t s = (int64_t)nk * (nk + 3) / 2;
+    if (L > K) s += (int64_t)(K + 1) * (L - K);
+    return s;
+}
 
 __host__ __device__ inline int64_t ax_block(int k, int Lt, int Ls) {
     int q = k < 0 ? -k : k;
@@ -133,12 +131,30 @@ __host__ __device__ inline int64_t ax_off(int k, int Lt, int Ls, int K) {
 __host__ __device__ inline int64_t ax_size(int Lt, int Ls, int K) { return ax_off(K, Lt, Ls, K) + 2 * ax_block(K, Lt, Ls); }
 __host__ __device__ inline int64_t sect_size(int K, int nmax) { return (int64_t)(K + 1) * (nmax + 2); }
 
+// Coaxial tables by Gauss-Legendre quadrature (ifs_coax_gq.hip): one table of
+// (kind, t, Lt, Ls, K) in the layout of launch_gen_axial, generated in batches of
+// mb orders m within ws_bytes of workspace.
+constexpr int GQ_MAXB = 96;
+struct GqPlan {
+    int kind = 1;
+    double t = 0.0;
+    int Lt = 0, Ls = 0, K = 0;
+    int P = 0;                         // outgoing kind: Legendre degree of T
+    int nb = 1;                        // bands of T: band b holds p in [lo[b], lo[b+1])
+    int lo[GQ_MAXB] = {0};
+    int Q = 0, Nr = 0, mb = 1;         // nodes, Legendre rows, orders per batch
+    size_t ws_bytes = 0;
+};
+GqPlan gq_plan(int kind, double t, int Lt, int Ls, int K, size_t ws_cap_bytes);
+// mu, w: the Q Gauss-Legendre nodes and weights on the device
+void launch_gen_axial_gq(const GqPlan &g, const double *mu, const double *w, void *ws, double2 *wax, hipStream_t s);
+
 // Kernel launchers (ifs_kernels.hip).  All enqueue on `s`.
-void launch_gen_wigner(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s);
 void launch_gen_wigner_fold(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s);
 void launch_gen_sect(const Task *tasks, int ntasks, int max_nmax, double2 *wsect, hipStream_t s);
 void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect, double2 *wax,
                       hipStream_t s);
+// max_rows: largest frow_size(Ls, K) of the tasks
 void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls, const double2 *src_pool,
                     const double *wd, double2 *wy, hipStream_t s);
 void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const double2 *wax, const double2 *wy,

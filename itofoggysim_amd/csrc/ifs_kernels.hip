@@ -2,6 +2,7 @@
 //
 // Translation pipeline per task (source expansion -> target expansion), all FP64:
 //   gen_wigner : Wigner d^n_{km}(theta) tables by the three-term recurrence in n
+//                (folded E/O tables for the rotations, see ifs_internal.h)
 //   gen_sect   : sectorial scalar coaxial coefficients (recurrence in m)
 //   gen_axial  : vector coaxial coefficients A_k(n,nu), B_k(n,nu) (recurrence in nu)
 //   rot_fwd    : y_+- = d(theta) diag(e^{i m phi}) (x_a +- x_b)   (forward rotation)
@@ -63,61 +64,6 @@ __device__ __forceinline__ double coef_b(int n, int m) {
 // ------------------------------------------------------------------------
 // Wigner d tables
 // ------------------------------------------------------------------------
-// One workgroup strip per task (grid.y), threads over (k,m) pairs.
-__global__ void k_gen_wigner(const Task *__restrict__ tasks, double *__restrict__ wd) {
-    const Task T = tasks[blockIdx.y];
-    const int K = T.K;
-    const int L = T.Ls > T.Lt ? T.Ls : T.Lt;
-    const int width = 2 * L + 1;
-    const int npairs = (2 * K + 1) * width;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    const int k = p / width - K;
-    const int m = p % width - L;
-    const int ak = k < 0 ? -k : k, am = m < 0 ? -m : m;
-    const int n0 = ak > am ? ak : am;
-    const double cb = T.cth;
-    const double sb = sqrt(fmax(0.0, 1.0 - cb * cb));
-    // half-angle functions from cos(beta), beta in [0, pi]
-    const double ch = sqrt(fmax(0.0, 0.5 * (1.0 + cb)));
-    const double sh = sqrt(fmax(0.0, 0.5 * (1.0 - cb)));
-    (void)sb;
-    double *base = wd + T.off_d;
-    double dm2, dm1;
-    int nstart;
-    if (n0 == 0) {
-        dm2 = 1.0;
-        dm1 = cb;  // d^1_{00}
-        if (L >= 1) base[wig_off(1, K) + (0 + (1 < K ? 1 : K)) * 3 + 1] = cb;
-        nstart = 2;
-    } else {
-        dm2 = 0.0;
-        dm1 = wigner_seed(k, m, ch, sh);
-        int kk = n0 < K ? n0 : K;
-        base[wig_off(n0, K) + (int64_t)(k + kk) * (2 * n0 + 1) + (m + n0)] = dm1;
-        nstart = n0 + 1;
-    }
-    const double km = (double)k * (double)m;
-    const double k2 = (double)k * k, m2 = (double)m * m;
-    for (int n = nstart; n <= L; ++n) {
-        double nn = (double)n;
-        double num = (2.0 * nn - 1.0) * (nn * (nn - 1.0) * cb - km) * dm1 -
-                     nn * sqrt(((nn - 1.0) * (nn - 1.0) - k2) * ((nn - 1.0) * (nn - 1.0) - m2)) * dm2;
-        double den = (nn - 1.0) * sqrt((nn * nn - k2) * (nn * nn - m2));
-        double dn = num / den;
-        int kk = n < K ? n : K;
-        base[wig_off(n, K) + (int64_t)(k + kk) * (2 * n + 1) + (m + n)] = dn;
-        dm2 = dm1;
-        dm1 = dn;
-    }
-}
-
-void launch_gen_wigner(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s) {
-    if (ntasks <= 0) return;
-    dim3 grid((maxpairs + 255) / 256, ntasks);
-    hipLaunchKernelGGL(k_gen_wigner, grid, dim3(256), 0, s, tasks, wd);
-}
-
 // Folded tables (layout at fold_off): threads over (k, m), k = 0..K, m = 0..L; each
 // runs the recurrences of d_{k,m} and d_{k,-m} side by side from n0 = max(k, m).
 __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__restrict__ wd) {
@@ -307,27 +253,69 @@ __global__ void k_gen_axial(const Task *__restrict__ tasks, int max_nmax, const 
     double2 *An = wax + T.off_ax + ax_off(-m, Lt, Ls, K);
     double2 *Bn = An + ax_block(-m, Lt, Ls);
     const int ncol = Ls - q + 1;
+    // A/B of target order n, source order nu from alpha(n-1|n|n+1, nu)
+    auto emit = [&](int n, int nu, double2 c0, double2 cp, double2 cm) {
+        const double nn = (double)n;
+        const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1));
+        double2 s = cscale(sqrt(nn * (nn + 1.0)), c0);
+        s = cadd(s, cscale(t * coef_a(n, m) * sqrt(nn / (nn + 1.0)), cp));
+        s = cadd(s, cscale(t * coef_a(n - 1, m) * sqrt((nn + 1.0) / nn), cm));
+        double2 av = ipow_mul((nu - n) & 3, cscale(inv_nu, s));
+        double2 bv = ipow_mul((nu - n + 1) & 3, cscale(t * (double)m * inv_nu / sqrt(nn * (nn + 1.0)), c0));
+        int64_t o = (int64_t)(n - q) * ncol + (nu - q);
+        double2 pv = cadd(av, bv), qv = csub(av, bv);
+        Ap[o] = pv;  // P_m = A_m + B_m
+        Bp[o] = qv;  // Q_m = A_m - B_m
+        if (m > 0) {  // order -m: A unchanged, B negated
+            An[o] = qv;
+            Bn[o] = pv;
+        }
+    };
+    if (Ls > Lt) {
+        // The recurrence advances the column index and its error grows with the
+        // number of steps, so it runs over the smaller order: columns c = m..Lt+1
+        // (target orders) with the source orders as rows, and
+        //   alpha(n, nu) = (-1)^{n+nu} alpha(nu, n).
+        for (int c = m;; ++c) {
+            if (c <= Lt) {
+                const double ac = coef_a(c, m);
+                const double acm1 = coef_a(c - 1, m);
+                for (int r = m + threadIdx.x; r <= top - 1; r += blockDim.x) {
+                    double2 v = cscale(acm1, P[r]);
+                    v = csub(v, cscale(coef_a(r, m), C[r + 1]));
+                    if (r - 1 >= m) v = cadd(v, cscale(coef_a(r - 1, m), C[r - 1]));
+                    N[r] = cscale(1.0 / ac, v);
+                }
+                for (int r = top + threadIdx.x; r <= nmax + 1; r += blockDim.x) N[r] = make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+            if (c >= q && c <= Lt) {
+                const int n = c;
+                for (int nu = q + threadIdx.x; nu <= Ls; nu += blockDim.x) {
+                    const double sg = ((n + nu) & 1) ? -1.0 : 1.0;  // (-1)^{n+nu}
+                    double2 c0 = cscale(sg, C[nu]);
+                    double2 cp = cscale(-sg, N[nu]);
+                    double2 cm = (n - 1 >= m) ? cscale(-sg, P[nu]) : make_double2(0.0, 0.0);
+                    emit(n, nu, c0, cp, cm);
+                }
+            }
+            if (c >= Lt) break;
+            __syncthreads();
+            double2 *tmp = P;
+            P = C;
+            C = N;
+            N = tmp;
+            top -= 1;
+        }
+        return;
+    }
     for (int nu = m; nu <= Ls; ++nu) {
         if (nu >= 1) {
-            const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1));
             for (int n = q + threadIdx.x; n <= Lt; n += blockDim.x) {
-                double nn = (double)n;
                 double2 c0 = C[n];
                 double2 cp = C[n + 1];
                 double2 cm = (n - 1 >= m) ? C[n - 1] : make_double2(0.0, 0.0);
-                double2 s = cscale(sqrt(nn * (nn + 1.0)), c0);
-                s = cadd(s, cscale(t * coef_a(n, m) * sqrt(nn / (nn + 1.0)), cp));
-                s = cadd(s, cscale(t * coef_a(n - 1, m) * sqrt((nn + 1.0) / nn), cm));
-                double2 av = ipow_mul((nu - n) & 3, cscale(inv_nu, s));
-                double2 bv = ipow_mul((nu - n + 1) & 3, cscale(t * (double)m * inv_nu / sqrt(nn * (nn + 1.0)), c0));
-                int64_t o = (int64_t)(n - q) * ncol + (nu - q);
-                double2 pv = cadd(av, bv), qv = csub(av, bv);
-                Ap[o] = pv;  // P_m = A_m + B_m
-                Bp[o] = qv;  // Q_m = A_m - B_m
-                if (m > 0) {  // order -m: A unchanged, B negated
-                    An[o] = qv;
-                    Bn[o] = pv;
-                }
+                emit(n, nu, c0, cp, cm);
             }
         }
         if (nu == Ls) break;
@@ -360,60 +348,85 @@ void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, con
 }
 
 // ------------------------------------------------------------------------
-// Forward rotation: y(nu,k) = sum_m d^nu_{km} e^{i m phi} x(nu,m), |k| <= min(nu,K)
+// Forward rotation with the folded tables (E, O at fold_off, see ifs_internal.h):
+//   S_k = sum_{m>=0} E[k][m] s_m,  T_k = sum_{m>=0} O[k][m] t_m,  k = 0..min(nu,K)
+//   s_m = u_m + (-1)^m u_{-m} (s_0 = u_0), t_m = u_m - (-1)^m u_{-m}, u_m = e^{i m phi} x_m
+//   y_k = (S_k + T_k)/2,  y_{-k} = (-1)^k (S_k - T_k)/2
+// for both parts x_+- = x_a +- x_b.  One thread per (nu, k >= 0).
 // ------------------------------------------------------------------------
 __global__ void k_rot_fwd(const Task *__restrict__ tasks, const double2 *__restrict__ src_pool,
                           const double *__restrict__ wd, double2 *__restrict__ wy) {
-    extern __shared__ double2 ph[];
+    extern __shared__ double2 ph[];  // e^{i m phi}, m = 0..Ls
     const Task T = tasks[blockIdx.y];
     const int Ls = T.Ls, K = T.K;
-    for (int m = threadIdx.x; m <= 2 * Ls; m += blockDim.x) {
+    for (int m = threadIdx.x; m <= Ls; m += blockDim.x) {
         double sn, cs;
-        sincos((double)(m - Ls) * T.phi, &sn, &cs);
+        sincos((double)m * T.phi, &sn, &cs);
         ph[m] = make_double2(cs, sn);
     }
     __syncthreads();
-    const int64_t nent = rows_size(Ls, K);
+    const int64_t nf = frow_size(Ls, K);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nent) return;
+    if (e >= nf) return;
     int nu, k;
     {
-        int64_t lim = (int64_t)(K + 1) * (K + 1) - 1;
+        int64_t lim = frow_size(K, K);  // rows of degrees 1..K
         if (e < lim) {
-            nu = (int)sqrt((double)(e + 1));
-            while ((int64_t)(nu + 1) * (nu + 1) - 1 <= e) ++nu;
-            while ((int64_t)nu * nu - 1 > e) --nu;
+            nu = (int)((sqrt(8.0 * (double)e + 9.0) - 3.0) * 0.5);  // nu(nu+3)/2 <= e
+            while ((int64_t)(nu + 1) * (nu + 4) / 2 <= e) ++nu;
+            while (nu > 0 && (int64_t)nu * (nu + 3) / 2 > e) --nu;
+            k = (int)(e - (int64_t)nu * (nu + 3) / 2);
+            nu += 1;
         } else {
-            nu = K + 1 + (int)((e - lim) / (2 * K + 1));
+            nu = K + 1 + (int)((e - lim) / (K + 1));
+            k = (int)((e - lim) % (K + 1));
         }
-        int kk = nu < K ? nu : K;
-        k = (int)(e - rows_off(nu, K)) - kk;
     }
     const int kk = nu < K ? nu : K;
-    const double *drow = wd + T.off_d + wig_off(nu, K) + (int64_t)(k + kk) * (2 * nu + 1);
-    const double2 *xa = src_pool + T.src + fidx(nu, 0) - nu;
+    const double *E = wd + T.off_d + fold_off(nu, K) + (int64_t)k * (nu + 1);
+    const double *O = E + (int64_t)(kk + 1) * (nu + 1);
+    const double2 *xa = src_pool + T.src + fidx(nu, 0);  // xa[m], m = -nu..nu
     const double2 *xb = xa + (int64_t)Ls * (Ls + 2);
-    double2 sa = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
-    for (int m = -nu; m <= nu; ++m) {
-        double d = drow[m + nu];
-        double2 e_ = ph[m + Ls];
-        double2 va = cmul(e_, xa[m + nu]);
-        double2 vb = cmul(e_, xb[m + nu]);
-        sa.x += d * va.x;
-        sa.y += d * va.y;
-        sb.x += d * vb.x;
-        sb.y += d * vb.y;
+    double2 Sp = make_double2(0.0, 0.0), Sm = Sp, Tp = Sp, Tm = Sp;
+    for (int m = 0; m <= nu; ++m) {
+        const double2 p = ph[m];
+        double2 ap = xa[m], bp = xb[m];
+        double2 xp = cadd(ap, bp), xm = csub(ap, bp);  // x_+, x_- at m
+        double2 up = cmul(p, xp), um = cmul(p, xm);
+        const double ev = E[m], ov = O[m];
+        if (m == 0) {
+            Sp = cadd(Sp, cscale(ev, up));
+            Sm = cadd(Sm, cscale(ev, um));
+            continue;
+        }
+        const double2 pc = make_double2(p.x, -p.y);
+        double2 an = xa[-m], bn = xb[-m];
+        double2 vp = cmul(pc, cadd(an, bn)), vm = cmul(pc, csub(an, bn));  // u_{-m}
+        const double sg = (m & 1) ? -1.0 : 1.0;
+        Sp = cadd(Sp, cscale(ev, cadd(up, cscale(sg, vp))));
+        Sm = cadd(Sm, cscale(ev, cadd(um, cscale(sg, vm))));
+        Tp = cadd(Tp, cscale(ov, csub(up, cscale(sg, vp))));
+        Tm = cadd(Tm, cscale(ov, csub(um, cscale(sg, vm))));
     }
+    const int64_t nent = rows_size(Ls, K);
     double2 *y = wy + T.off_y;
-    y[e] = cadd(sa, sb);       // y_+
-    y[e + nent] = csub(sa, sb);  // y_-
+    const int64_t rp = rows_off(nu, K) + kk + k;
+    y[rp] = cscale(0.5, cadd(Sp, Tp));            // y_+ at (nu, k)
+    y[rp + nent] = cscale(0.5, cadd(Sm, Tm));     // y_- at (nu, k)
+    if (k > 0) {
+        const double f = (k & 1) ? -0.5 : 0.5;
+        const int64_t rm = rows_off(nu, K) + kk - k;
+        y[rm] = cscale(f, csub(Sp, Tp));
+        y[rm + nent] = cscale(f, csub(Sm, Tm));
+    }
 }
 
 void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls, const double2 *src_pool,
                     const double *wd, double2 *wy, hipStream_t s) {
     if (ntasks <= 0) return;
+    // max_rows bounds the folded rows (about half the unfolded ones)
     dim3 grid((unsigned)((max_rows + 255) / 256), ntasks);
-    size_t shm = sizeof(double2) * (size_t)(2 * max_Ls + 1);
+    size_t shm = sizeof(double2) * (size_t)(max_Ls + 1);
     hipLaunchKernelGGL(k_rot_fwd, grid, dim3(256), shm, s, tasks, src_pool, wd, wy);
 }
 
@@ -468,8 +481,13 @@ void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const d
 }
 
 // ------------------------------------------------------------------------
-// Back rotation and accumulation into the target, one workgroup per target segment:
-//   out(n,m) += e^{-i m phi} sum_k d^n_{km} z(n,k)
+// Back rotation with the folded tables and accumulation into the target, one
+// workgroup per target segment (fixed task order: deterministic, no atomics):
+//   F_m = sum_{k>=0} E[k][m] s_k,  G_m = sum_{k>=0} O[k][m] t_k,  m = 0..n
+//   s_k = z_k + (-1)^k z_{-k} (s_0 = z_0), t_k = z_k - (-1)^k z_{-k}
+//   w_m = (F_m + G_m)/2,  w_{-m} = (-1)^m (F_m - G_m)/2
+//   out_a/b(n, +-m) += e^{-+ i m phi} (w_+ +- w_-)/2
+// One thread per (n, m >= 0).
 // ------------------------------------------------------------------------
 __global__ void k_rot_back(const Task *__restrict__ tasks, const Segment *__restrict__ segs,
                            const double *__restrict__ wd, const double2 *__restrict__ wz,
@@ -477,38 +495,60 @@ __global__ void k_rot_back(const Task *__restrict__ tasks, const Segment *__rest
     const Segment S = segs[blockIdx.x];
     const int Lt = S.Lt;
     const int nent = Lt * (Lt + 2);
-    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < nent; e += blockDim.x * gridDim.y) {
-        int n = (int)sqrt((double)(e + 1));
-        while ((n + 1) * (n + 1) - 1 <= e) ++n;
-        while (n * n - 1 > e) --n;
-        const int m = e - (n * n - 1) - n;
-        double2 oa = make_double2(0.0, 0.0), ob = make_double2(0.0, 0.0);
+    const int nf = Lt * (Lt + 3) / 2;  // entries (n, m >= 0), n = 1..Lt
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < nf; e += blockDim.x * gridDim.y) {
+        int n = (int)((sqrt(8.0 * (double)e + 9.0) - 3.0) * 0.5);
+        while ((n + 1) * (n + 4) / 2 <= e) ++n;
+        while (n > 0 && n * (n + 3) / 2 > e) --n;
+        const int m = e - n * (n + 3) / 2;
+        n += 1;
+        double2 ap = make_double2(0.0, 0.0), bp = ap, an = ap, bn = ap;  // target increments at +m, -m
         for (int ti = S.t_begin; ti < S.t_end; ++ti) {
             const Task T = tasks[ti];
             const int K = T.K;
             const int kk = n < K ? n : K;
-            const double *dcol = wd + T.off_d + wig_off(n, K) + (m + n);
+            const double *E = wd + T.off_d + fold_off(n, K) + m;  // E[k][m] at k * (n+1)
+            const double *O = E + (int64_t)(kk + 1) * (n + 1);
             const int64_t nz = rows_size(Lt, K);
-            const double2 *za = wz + T.off_z + rows_off(n, K) + kk;
-            const double2 *zb = za + nz;
-            double2 sa = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
-            for (int k = -kk; k <= kk; ++k) {
-                double d = dcol[(int64_t)(k + kk) * (2 * n + 1)];
-                sa.x += d * za[k].x;
-                sa.y += d * za[k].y;
-                sb.x += d * zb[k].x;
-                sb.y += d * zb[k].y;
+            const double2 *zp = wz + T.off_z + rows_off(n, K) + kk;  // z_+(n, k) at zp[k]
+            const double2 *zm = zp + nz;                              // z_-(n, k)
+            double2 Fp = make_double2(0.0, 0.0), Fm = Fp, Gp = Fp, Gm = Fp;
+            for (int k = 0; k <= kk; ++k) {
+                const double ev = E[(int64_t)k * (n + 1)], ov = O[(int64_t)k * (n + 1)];
+                if (k == 0) {
+                    Fp = cadd(Fp, cscale(ev, zp[0]));
+                    Fm = cadd(Fm, cscale(ev, zm[0]));
+                    continue;
+                }
+                const double sg = (k & 1) ? -1.0 : 1.0;
+                Fp = cadd(Fp, cscale(ev, cadd(zp[k], cscale(sg, zp[-k]))));
+                Fm = cadd(Fm, cscale(ev, cadd(zm[k], cscale(sg, zm[-k]))));
+                Gp = cadd(Gp, cscale(ov, csub(zp[k], cscale(sg, zp[-k]))));
+                Gm = cadd(Gm, cscale(ov, csub(zm[k], cscale(sg, zm[-k]))));
             }
             double sn, cs;
-            sincos(-(double)m * T.phi, &sn, &cs);
-            double2 e_ = make_double2(cs, sn);
-            oa = cadd(oa, cmul(e_, sa));
-            ob = cadd(ob, cmul(e_, sb));
+            sincos((double)m * T.phi, &sn, &cs);
+            const double2 pc = make_double2(cs, -sn), p = make_double2(cs, sn);  // e^{-i m phi}, e^{i m phi}
+            // w at +m: (F + G)/2 per part; a = (w_+ + w_-)/2, b = (w_+ - w_-)/2
+            double2 wpp = cscale(0.5, cadd(Fp, Gp)), wpm = cscale(0.5, cadd(Fm, Gm));
+            ap = cadd(ap, cmul(pc, cscale(0.5, cadd(wpp, wpm))));
+            bp = cadd(bp, cmul(pc, cscale(0.5, csub(wpp, wpm))));
+            if (m > 0) {
+                const double f = (m & 1) ? -0.5 : 0.5;
+                double2 wnp = cscale(f, csub(Fp, Gp)), wnm = cscale(f, csub(Fm, Gm));
+                an = cadd(an, cmul(p, cscale(0.5, cadd(wnp, wnm))));
+                bn = cadd(bn, cmul(p, cscale(0.5, csub(wnp, wnm))));
+            }
         }
-        // oa, ob hold the back-rotated z_+, z_-: a = (z_+ + z_-)/2, b = (z_+ - z_-)/2
         double2 *dst = dst_pool + S.dst;
-        dst[e] = cadd(dst[e], cscale(0.5, cadd(oa, ob)));
-        dst[e + nent] = cadd(dst[e + nent], cscale(0.5, csub(oa, ob)));
+        const int o = n * n - 1 + m + n;
+        dst[o] = cadd(dst[o], ap);
+        dst[o + nent] = cadd(dst[o + nent], bp);
+        if (m > 0) {
+            const int on = n * n - 1 - m + n;
+            dst[on] = cadd(dst[on], an);
+            dst[on + nent] = cadd(dst[on + nent], bn);
+        }
     }
 }
 

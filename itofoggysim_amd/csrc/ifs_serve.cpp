@@ -82,7 +82,7 @@ size_t estimate_bytes(const Req &r) {
         for (int b = 0; b < ns; ++b) {
             if (a == b) continue;
             double Ls = L[a], Lt = L[b], K = std::min(Ls, Lt), Lm = std::max(Ls, Lt);
-            tab += 8.0 * (2 * K + 1) * (Lm + 1) * (Lm + 1) + 16.0 * 2.0 * (2 * K + 1) * Ls * Lt +
+            tab += 8.0 * 2.0 * (K + 1) * (Lm + 1) * (Lm + 2) / 2.0 + 16.0 * 2.0 * (2 * K + 1) * Ls * Lt +
                    16.0 * 2.0 * (2 * K + 1) * (Ls + Lt);
         }
     return (size_t)(bytes + tab) + ((size_t)64 << 20);

@@ -17,7 +17,7 @@ from ._lib import check, lib
 class MultiSphereSolver:
     def __init__(self, centres, radii, m_rel, k, n_global=None, lmax=None, z_selector=3, ref_row_zeroing=False,
                  use_fmm=False, leaf_size=32, fmm_d0=6.0, fmm_lmax_cap=0, workspace_mb=0,
-                 d_convention="physical"):
+                 d_convention="physical", cache_mb=0):
         centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
         radii = np.ascontiguousarray(radii, dtype=np.float64)
         ns = radii.shape[0]
@@ -43,6 +43,7 @@ class MultiSphereSolver:
         if d_convention not in ("physical", "reference"):
             raise ValueError("d_convention must be 'physical' or 'reference'")
         pr.d_convention = 0 if d_convention == "physical" else 1
+        pr.cache_mb = int(cache_mb)
         h = ctypes.c_void_p()
         check(lib().ifs_create(ctypes.byref(h), ctypes.byref(cl), ctypes.byref(pr)), "ifs_create")
         self._h = h

@@ -13,19 +13,27 @@ def _rand(n, seed):
     return rng.standard_normal(n) + 1j * rng.standard_normal(n)
 
 
-@pytest.mark.parametrize("ns,s,d0,cap,seed,conv", [(24, 2, 6.0, 0, 11, "physical"), (40, 1, 3.0, 0, 12, "physical"),
-                                                    (30, 2, 6.0, 5, 13, "physical"), (24, 2, 6.0, 0, 14, "reference")])
-def test_fmm_vs_oracle_fmm(oracle, ns, s, d0, cap, seed, conv):
+@pytest.mark.parametrize("ns,s,d0,cap,seed,conv,cache", [(24, 2, 6.0, 0, 11, "physical", 0),
+                                                          (40, 1, 3.0, 0, 12, "physical", 0),
+                                                          (30, 2, 6.0, 5, 13, "physical", 0),
+                                                          (24, 2, 6.0, 0, 14, "reference", 0),
+                                                          (40, 1, 3.0, 0, 12, "physical", -1),
+                                                          (24, 2, 6.0, 0, 14, "reference", -1)])
+def test_fmm_vs_oracle_fmm(oracle, ns, s, d0, cap, seed, conv, cache):
+    """cache 0: translation tables kept across matvecs (generated in the first);
+    cache -1: regenerated in every matvec in shared workspace slots."""
     from oracle import fmm_ref
     rng = np.random.default_rng(seed)
     pos = rng.uniform(0.0, 8.0, (ns, 3))
     lmax = rng.integers(1, 4, ns).astype(np.int32)
     k = 0.6
     solver = MultiSphereSolver(pos, np.full(ns, 0.2), 1.5 + 0.01j, k, lmax=lmax, use_fmm=True, leaf_size=s,
-                               fmm_d0=d0, fmm_lmax_cap=cap, d_convention=conv)
+                               fmm_d0=d0, fmm_lmax_cap=cap, d_convention=conv, cache_mb=cache)
     an, bn = solver.mie_coefficients()
     x = _rand(solver.n, seed)
     b = solver.calculate_vector_b(x)
+    b2 = solver.calculate_vector_b(x)  # kept (or regenerated) tables: the same result bit for bit
+    assert np.array_equal(b, b2)
     dsign = -1.0 if conv == "physical" else 1.0
     ref, tr = fmm_ref.fmm_matvec(pos, lmax, solver.n_global, an, bn, k, x, s=s, d0=d0, cap=cap, dsign=dsign)
     u, lmin, lm = solver.tree_export()
@@ -69,3 +77,26 @@ def test_fmm_chunked_workspace_config1():
     assert err < 1e-13
     a.close()
     b.close()
+
+
+def test_fmm_table_cache_modes_config1():
+    """Tables kept across matvecs vs regenerated per matvec in workspace slots, with the
+    classes cut into many chunks: the same products, so the same result to rounding of
+    the summation order (none here: identical kernels and order)."""
+    c = config_cluster(1)
+    kw = dict(lmax=c["lmax"], use_fmm=True, leaf_size=2, fmm_d0=6.0)
+    a = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
+    b = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], cache_mb=-1, **kw)
+    d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], cache_mb=-1, workspace_mb=1, **kw)
+    x = _rand(a.n, 6)
+    ya = a.calculate_vector_b(x)
+    yb = b.calculate_vector_b(x)
+    yd = d.calculate_vector_b(x)
+    assert np.array_equal(ya, a.calculate_vector_b(x))
+    assert np.array_equal(yb, b.calculate_vector_b(x))
+    assert np.array_equal(ya, yb)
+    err = np.linalg.norm(ya - yd) / np.linalg.norm(ya - x)
+    print("kept vs regenerated tables, chunked", err)
+    assert err < 1e-13
+    for s_ in (a, b, d):
+        s_.close()
