@@ -68,7 +68,7 @@ typedef struct {
                                 values split the translation classes into more workspace chunks */
     int32_t d_convention;    /* IFS_D_PHYSICAL (default) or IFS_D_REFERENCE */
     int32_t cache_mb;        /* device memory for translation tables kept across matvecs, in MiB
-                                (0: 40% of free memory at creation; < 0: none, tables are
+                                (0: 65% of free memory at creation; < 0: none, tables are
                                 regenerated in every matvec) */
     int32_t reserved;
 } ifs_params;

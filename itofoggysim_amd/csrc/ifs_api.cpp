@@ -296,6 +296,8 @@ static bool same_ax(const Task &a, const Task &b) {
     return a.t == b.t && a.kind == b.kind && a.Ls == b.Ls && a.Lt == b.Lt && a.K == b.K;
 }
 
+static int class_batch_keys(ifs_ctx *c, ClassBatch &B);
+
 int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
     const int nc = (int)B.gen.size();
     B.chunks.clear();
@@ -347,7 +349,7 @@ int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
             B.phase[(size_t)i * PW + m + B.PL] = make_double2((double)cosl(a), (double)sinl(a));
         }
     B.plan_nrhs = 0;
-    return class_batch_plan(c, B, 1);
+    return class_batch_keys(c, B);
 }
 
 // Chunks, runs, segments and reduction lists for nrhs right-hand sides per call.
@@ -394,6 +396,92 @@ int gq_rule(ifs_ctx *c, int Q, const double **mu, const double **w) {
     }
     *mu = it->second.as<double>();
     *w = it->second.as<double>() + Q;
+    return IFS_OK;
+}
+
+// Table keys of a finalized class batch: one Wigner table per |cos(theta)|, one
+// coaxial table per (length, kind), with the quadrature plan of each coaxial table.
+static int class_batch_keys(ifs_ctx *c, ClassBatch &B) {
+    const Task &T0 = B.gen[0];
+    B.n_d = fold_size(std::max(T0.Ls, T0.Lt), T0.K);
+    B.n_s = sect_size(T0.K, T0.Lt + T0.Ls + 2);
+    B.n_ax = ax_size(T0.Lt, T0.Ls, T0.K);
+    // table keys (Ls, Lt, K are common to the batch)
+    {
+        std::map<double, int> dk;
+        std::map<std::pair<double, int>, int> ak;
+        B.dkey.assign(B.gen.size(), 0);
+        B.axkey.assign(B.gen.size(), 0);
+        B.dkeys.clear();
+        B.axkey_cls.clear();
+        for (size_t g = 0; g < B.gen.size(); ++g) {
+            const Task &T = B.gen[g];
+            const double a = std::fabs(T.cth);
+            auto it = dk.find(a);
+            if (it == dk.end()) {
+                it = dk.emplace(a, (int)B.dkeys.size()).first;
+                Task K = T;
+                K.cth = a;
+                B.dkeys.push_back(K);
+            }
+            B.dkey[g] = it->second;
+            auto jt = ak.find({T.t, T.kind});
+            if (jt == ak.end()) {
+                jt = ak.emplace(std::make_pair(T.t, T.kind), (int)B.axkey_cls.size()).first;
+                B.axkey_cls.push_back((int)g);
+            }
+            B.axkey[g] = jt->second;
+        }
+    }
+    // coaxial tables by quadrature (stable at every order)
+    B.gq.clear();
+    for (int g : B.axkey_cls) {
+        const Task &T = B.gen[g];
+        B.gq.push_back(gq_plan(T.kind, T.t, T0.Lt, T0.Ls, T0.K, (size_t)1 << 30));
+        const double *mu, *w;
+        int r = gq_rule(c, B.gq.back().Q, &mu, &w);
+        if (r) return r;
+    }
+    return IFS_OK;
+}
+
+// Tables kept across matvecs, decided over all class batches at once: coaxial
+// tables first (largest first: they are the dearest to regenerate), then Wigner
+// tables, each batch's set only as a whole and only while the cache budget lasts.
+int class_batches_prepare(ifs_ctx *c, const std::vector<ClassBatch *> &bs) {
+    std::vector<ClassBatch *> ord;
+    for (auto *B : bs)
+        if (!B->gen.empty()) ord.push_back(B);
+    auto ax_bytes = [](const ClassBatch *B) { return (size_t)B->n_ax * 16 * B->axkey_cls.size(); };
+    auto d_bytes = [](const ClassBatch *B) { return (size_t)B->n_d * 8 * B->dkeys.size(); };
+    std::stable_sort(ord.begin(), ord.end(), [&](ClassBatch *a, ClassBatch *b) { return ax_bytes(a) > ax_bytes(b); });
+    for (auto *B : ord) {
+        const size_t need = ax_bytes(B);
+        if (B->ax_persist || !need || c->ws_cached + need > c->cache_budget) continue;
+        if (B->own_ax.alloc(need) != IFS_OK) {
+            (void)hipGetLastError();
+            continue;
+        }
+        c->ws_cached += need;
+        B->ax_persist = true;
+        B->ax_ready.assign(B->axkey_cls.size(), 0);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](ClassBatch *a, ClassBatch *b) { return d_bytes(a) > d_bytes(b); });
+    for (auto *B : ord) {
+        const size_t need = d_bytes(B);
+        if (B->d_persist || !need || c->ws_cached + need > c->cache_budget) continue;
+        if (B->own_d.alloc(need) != IFS_OK) {
+            (void)hipGetLastError();
+            continue;
+        }
+        c->ws_cached += need;
+        B->d_persist = true;
+        B->d_ready.assign(B->dkeys.size(), 0);
+    }
+    for (auto *B : ord) {
+        int rc = class_batch_plan(c, *B, 1);
+        if (rc) return rc;
+    }
     return IFS_OK;
 }
 
@@ -497,66 +585,9 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     (void)nbox;
     const int L = std::max(T0.Ls, T0.Lt);
     B.n_d = fold_size(L, T0.K);
-    B.n_s = sect_size(T0.K, T0.Lt + T0.Ls + 2);
-    B.n_ax = ax_size(T0.Lt, T0.Ls, T0.K);
-    // table keys (Ls, Lt, K are common to the batch)
-    {
-        std::map<double, int> dk;
-        std::map<std::pair<double, int>, int> ak;
-        B.dkey.assign(B.gen.size(), 0);
-        B.axkey.assign(B.gen.size(), 0);
-        B.dkeys.clear();
-        B.axkey_cls.clear();
-        for (size_t g = 0; g < B.gen.size(); ++g) {
-            const Task &T = B.gen[g];
-            const double a = std::fabs(T.cth);
-            auto it = dk.find(a);
-            if (it == dk.end()) {
-                it = dk.emplace(a, (int)B.dkeys.size()).first;
-                Task K = T;
-                K.cth = a;
-                B.dkeys.push_back(K);
-            }
-            B.dkey[g] = it->second;
-            auto jt = ak.find({T.t, T.kind});
-            if (jt == ak.end()) {
-                jt = ak.emplace(std::make_pair(T.t, T.kind), (int)B.axkey_cls.size()).first;
-                B.axkey_cls.push_back((int)g);
-            }
-            B.axkey[g] = jt->second;
-        }
-    }
-    // coaxial tables by quadrature (stable at every order)
-    B.gq.clear();
-    size_t gq_ws = 0;
-    for (int g : B.axkey_cls) {
-        const Task &T = B.gen[g];
-        B.gq.push_back(gq_plan(T.kind, T.t, T0.Lt, T0.Ls, T0.K, (size_t)1 << 30));
-        gq_ws = std::max(gq_ws, B.gq.back().ws_bytes);
-        const double *mu, *w;
-        int r = gq_rule(c, B.gq.back().Q, &mu, &w);
-        if (r) return r;
-    }
     int rc;
-    // persistent tables when the cache budget holds all of a kind (Wigner first)
-    if (!B.d_persist) {
-        const size_t need = (size_t)B.n_d * 8 * B.dkeys.size();
-        if (c->ws_cached + need <= c->cache_budget && need > 0) {
-            if ((rc = B.own_d.alloc(need)) != IFS_OK) return rc;
-            c->ws_cached += need;
-            B.d_persist = true;
-            B.d_ready.assign(B.dkeys.size(), 0);
-        }
-    }
-    if (!B.ax_persist) {
-        const size_t need = (size_t)B.n_ax * 16 * B.axkey_cls.size();
-        if (c->ws_cached + need <= c->cache_budget && need > 0) {
-            if ((rc = B.own_ax.alloc(need)) != IFS_OK) return rc;
-            c->ws_cached += need;
-            B.ax_persist = true;
-            B.ax_ready.assign(B.axkey_cls.size(), 0);
-        }
-    }
+    size_t gq_ws = 0;
+    for (auto &G : B.gq) gq_ws = std::max(gq_ws, G.ws_bytes);
     // otherwise Wigner table slots: the tables of a chunk's runs stay for the back pass
     // when a quarter of the workspace budget holds them (run r uses slot r % nslots)
     size_t maxruns = 1;
@@ -818,7 +849,7 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     (void)hipMemGetInfo(&freeb, &totb);
     c->ws_budget = std::min<size_t>(freeb / 4, (size_t)48 << 30);
     if (p->workspace_mb > 0) c->ws_budget = (size_t)p->workspace_mb << 20;
-    c->cache_budget = freeb / 5 * 2;
+    c->cache_budget = freeb / 20 * 13;  // 65%: the translation workspace takes at most 48 GiB of the rest
     if (p->cache_mb > 0) c->cache_budget = (size_t)p->cache_mb << 20;
     if (p->cache_mb < 0) c->cache_budget = 0;
     if (p->d_convention != IFS_D_PHYSICAL && p->d_convention != IFS_D_REFERENCE) return fail(IFS_E_ARG);

@@ -204,6 +204,8 @@ int class_batch_finalize(ifs_ctx *c, ClassBatch &B);
 // Device Gauss-Legendre rule of Q nodes (cached per context): mu, then w.
 int gq_rule(ifs_ctx *c, int Q, const double **mu, const double **w);
 int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs);
+// After every class batch is finalized: decides which tables persist, then plans each.
+int class_batches_prepare(ifs_ctx *c, const std::vector<ClassBatch *> &bs);
 int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs);
 void class_batch_cost(const ClassBatch &B, double &flops, double &bytes);
 

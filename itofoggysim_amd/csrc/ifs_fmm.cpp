@@ -294,12 +294,24 @@ int build(ifs_ctx *c, Fmm **out) {
         if (!r) class_batch_cost(B, c->flops_mv, c->bytes_mv);
         return r;
     };
-    if ((rc = fin(f->p2m, "p2m"))) goto fail;
-    for (int l = lmin; l <= lmax; ++l) {
-        if (l < lmax && (rc = cfin(f->m2m[l], "m2m"))) goto fail;
-        if ((rc = cfin(f->m2l[l], "m2l"))) goto fail;
-        if (l > lmin && (rc = cfin(f->l2l[l], "l2l"))) goto fail;
+    // class batches first: their tables have the first claim on the cache budget
+    {
+        std::vector<ClassBatch *> cls;
+        for (int l = lmin; l <= lmax; ++l) {
+            if (l < lmax) {
+                if ((rc = cfin(f->m2m[l], "m2m"))) goto fail;
+                cls.push_back(&f->m2m[l]);
+            }
+            if ((rc = cfin(f->m2l[l], "m2l"))) goto fail;
+            cls.push_back(&f->m2l[l]);
+            if (l > lmin) {
+                if ((rc = cfin(f->l2l[l], "l2l"))) goto fail;
+                cls.push_back(&f->l2l[l]);
+            }
+        }
+        if ((rc = class_batches_prepare(c, cls))) goto fail;
     }
+    if ((rc = fin(f->p2m, "p2m"))) goto fail;
     if ((rc = fin(f->l2p, "l2p"))) goto fail;
     if ((rc = fin(f->p2p, "p2p"))) goto fail;
     *out = f;

@@ -53,11 +53,14 @@ constexpr int RPAD = 2;  // LDS row padding (doubles)
 //   E(pi - b) = sigma E(b),  O(pi - b) = -sigma O(b),
 // so the column entries take (-1)^m (forward) or (-1)^k (back) at the LDS store and
 // the results (-1)^{n+k} (forward) or (-1)^{n+m} (back), with T and G negated.
-// Block: 64 rows of each
-// product x 128 real columns; wave w owns rows 32 (w / 2) .. + 31 and columns
-// 64 (w % 2) .. + 63 of both products, so S and T (F and G) of one entry meet in
-// the same lane.  Each gathered column element serves 64 rows of each table.
+// Back product block: 64 rows of each product x 128 real columns; wave w owns
+// rows 32 (w / 2) .. + 31 and columns 64 (w % 2) .. + 63 of both products, so F
+// and G of one entry meet in the same lane.  Forward product block: 32 rows of
+// each product x 128 columns; waves 0,1 compute E, waves 2,3 O (64 columns each),
+// and S and T meet through LDS in the epilogue (at 64 rows the forward kernel's
+// gather registers push it to one wave per SIMD, which is slower).
 constexpr int FM = 64;
+constexpr int FMF = 32;
 constexpr int FN = 128;
 constexpr int FK = 16;
 
@@ -77,26 +80,23 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     const int NTOT = C.ncols * C.nrhs;  // columns: translations x right-hand sides
     const int NC = 4 * NTOT;
     const int NCY = 4 * C.ncy * C.nrhs;
-    const int row0 = blockIdx.y * FM;
+    const int row0 = blockIdx.y * FMF;
     const int col0 = blockIdx.x * FN;
     if (row0 >= M) return;
-    __shared__ double As[FK][2 * FM + RPAD];     // columns 0..FM-1: E rows, FM..2FM-1: O rows
+    __shared__ double As[FK][2 * FMF + RPAD];     // columns 0..31: E rows, 32..63: O rows
     __shared__ double Bs[2][FK][FN + RPAD];      // s, t
     const double *E = wd + C.off_d + fold_off(nu, K);
     const double *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int wr = (w >> 1) * 32;    // wave rows
-    const int wc = (w & 1) * 64;     // wave columns
-    dv4 accS[2][4], accT[2][4];
+    const int g = w >> 1;            // 0: E product, 1: O product
+    const int wc = (w & 1) * 64;
+    dv4 acc[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            accS[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-            accT[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-        }
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
     const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
     // gather: thread -> (translation t, inner q), idx = tid + 256 e, t = idx / 16, q = idx % 16
     const double2 *xs[2];
@@ -110,15 +110,15 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         xs[e] = ok ? src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0) : nullptr;  // xs[m]
         ps[e] = ok ? phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL : nullptr;
     }
-    double ra[8];
+    double ra[4];
     double2 rap[2], rbp[2], ram[2], rbm[2], rph[2];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            int idx = tid + 256 * e;  // 2 FM rows (E FM, O FM) x 16 inner
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;  // 64 rows (E 32, O 32) x 16 inner
             int r = idx >> 4, q = idx & 15;
-            int gr = row0 + (r & (FM - 1)), gq = k0 + q;
-            const double *src = (r < FM) ? E : O;
+            int gr = row0 + (r & 31), gq = k0 + q;
+            const double *src = (r < 32) ? E : O;
             ra[e] = (gr < M && gq < KI) ? src[(int64_t)gr * KI + gq] : 0.0;
         }
 #pragma unroll
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     load(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < 4; ++e) {
             int idx = tid + 256 * e;
             As[idx & 15][idx >> 4] = ra[e];
         }
@@ -179,27 +179,30 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         if (k0 + FK < KI) load(k0 + FK);
 #pragma unroll
         for (int kc = 0; kc < FK; kc += 4) {
-            double ae[2], ao[2], bs[4], bt[4];
+            double a[2], b[4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ae[i] = As[kc + lk][wr + 16 * i + li];
-                ao[i] = As[kc + lk][FM + wr + 16 * i + li];
-            }
+            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                bs[j] = Bs[0][kc + lk][wc + 16 * j + li];
-                bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
-            }
+            for (int j = 0; j < 4; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    accS[i][j] = mfma(ae[i], bs[j], accS[i][j]);
-                    accT[i][j] = mfma(ao[i], bt[j], accT[i][j]);
-                }
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
+    // epilogue: the O-product waves hand T to the E-product waves through LDS
+    double *X = &Bs[0][0][0];  // 32 x (FN + RPAD) doubles
+    if (g == 1) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FN + RPAD) + wc + 16 * j + li] = acc[i][j][v];
+    }
+    __syncthreads();
+    if (g == 1) return;
     double *Y = wy + rows_off(nu, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -210,9 +213,10 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             if (gc >= NC) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int k = row0 + wr + 16 * i + 4 * v + lk;
+                int r = 16 * i + 4 * v + lk;
+                int k = row0 + r;
                 if (k >= M) continue;
-                double S = accS[i][j][v], T = accT[i][j][v];
+                double S = acc[i][j][v], T = X[r * (FN + RPAD) + c];
                 if (C.mirror) {
                     double f = ((nu + k) & 1) ? -1.0 : 1.0;
                     S *= f;
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         }
 }
 
-__global__ __launch_bounds__(256) void k_class_rot_back(const GroupDesc *__restrict__ desc,
+__global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__restrict__ desc,
                                                         const int32_t *__restrict__ pidx,
                                                         const double2 *__restrict__ phase,
                                                         const double *__restrict__ wd, const double *__restrict__ wz,
@@ -533,7 +537,7 @@ void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *sr
     const int K = h.K;
     if (h.ncols <= 0 || h.nrhs <= 0) return;
     if (stage == 0) {
-        int mt = ((h.Ls < K ? h.Ls : K) + 1 + FM - 1) / FM;
+        int mt = ((h.Ls < K ? h.Ls : K) + 1 + FMF - 1) / FMF;
         dim3 grid((NC + FN - 1) / FN, mt, h.Ls);
         hipLaunchKernelGGL(k_class_rot_fwd, grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
     } else if (stage == 1) {

@@ -64,7 +64,7 @@ namespace {
 
 // Device bytes a request needs: vectors, Krylov basis of the lock-step GMRES and
 // the direct-path translation tables (kept for the whole request).
-size_t estimate_bytes(const Req &r) {
+size_t estimate_bytes(const Req &r, size_t *tables = nullptr) {
     const int ns = (int)r.radii.size();
     int N = r.prm.n_global;
     const double k = r.prm.k;
@@ -85,6 +85,7 @@ size_t estimate_bytes(const Req &r) {
             tab += 8.0 * 2.0 * (K + 1) * (Lm + 1) * (Lm + 2) / 2.0 + 16.0 * 2.0 * (2 * K + 1) * Ls * Lt +
                    16.0 * 2.0 * (2 * K + 1) * (Ls + Lt);
         }
+    if (tables) *tables = (size_t)tab;
     return (size_t)(bytes + tab) + ((size_t)64 << 20);
 }
 
@@ -190,7 +191,10 @@ int ifs_server_submit(ifs_server *s, const ifs_cluster *cl, const ifs_params *pr
     r.prm = *prm;
     r.tol = tol;
     r.restart = restart;
-    r.footprint = estimate_bytes(r);
+    size_t tab = 0;
+    r.footprint = estimate_bytes(r, &tab);
+    // a request caches no more tables than its admitted footprint holds
+    if (r.prm.cache_mb == 0) r.prm.cache_mb = (int32_t)std::min<size_t>((size_t)1 << 30, (tab >> 20) + 1);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return IFS_E_ARG;
     r.id = s->next_id++;
