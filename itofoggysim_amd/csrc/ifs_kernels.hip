@@ -109,14 +109,24 @@ __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__rest
         put(n0, p1, q1);
         nstart = n0 + 1;
     }
+    // The three-term recurrence of wigner_step for both chains: they share
+    // sqrt((n^2 - k^2)(n^2 - m^2)), which also serves as the d^{n-2} coefficient of
+    // the next degree, so each degree costs one square root and one division.
+    const double k2 = (double)k * k, m2 = (double)m * m, km = (double)k * m;
+    double sprev = sqrt(fmax(0.0, ((nstart - 1.0) * (nstart - 1.0) - k2) * ((nstart - 1.0) * (nstart - 1.0) - m2)));
     for (int n = nstart; n <= L; ++n) {
-        double pn = wigner_step(n, k, m, cb, p1, p2);
-        double qn = (m == 0) ? pn : wigner_step(n, k, -m, cb, q1, q2);
+        const double nn = (double)n;
+        const double scur = sqrt((nn * nn - k2) * (nn * nn - m2));
+        const double inv = 1.0 / ((nn - 1.0) * scur);
+        const double a = (2.0 * nn - 1.0), b = nn * (nn - 1.0) * cb, c2 = nn * sprev;
+        const double pn = (a * (b - km) * p1 - c2 * p2) * inv;
+        const double qn = (m == 0) ? pn : (a * (b + km) * q1 - c2 * q2) * inv;
         put(n, pn, qn);
         p2 = p1;
         p1 = pn;
         q2 = q1;
         q1 = qn;
+        sprev = scur;
     }
 }
 

@@ -100,3 +100,41 @@ def test_fmm_table_cache_modes_config1():
     assert err < 1e-13
     for s_ in (a, b, d):
         s_.close()
+
+
+def _snm_flat(N, lmax):
+    """Per-entry factors s_nm of the normalised basis over the flat vector, and the
+    mask of the rows n <= L_j."""
+    from scipy.special import gammaln
+    nn = np.array([q for q in range(1, N + 1) for m in range(-q, q + 1)], dtype=float)
+    mm = np.array([m for q in range(1, N + 1) for m in range(-q, q + 1)], dtype=float)
+    s = np.exp(0.5 * (np.log(nn * (nn + 1) * (2 * nn + 1)) + gammaln(nn - mm + 1) - gammaln(nn + mm + 1)))
+    mask = np.concatenate([np.tile(nn <= min(l, N), 2) for l in lmax])
+    return np.tile(np.concatenate([s, s]), len(lmax)), mask
+
+
+@pytest.mark.parametrize("n", [200, 600])
+def test_fmm_vs_direct_high_order(n):
+    """Clusters of the 2000-sphere generator with n spheres: one MLFMM level with box
+    order 377 (n = 200) or 536 (n = 600), far beyond the orders where the coaxial
+    recurrences hold.  The MLFMM matvec must match the direct one, compared in the
+    normalised basis (the flat basis spans 1e+-80 at N = 49)."""
+    c = config_cluster(2, n=n)
+    kw = dict(lmax=c["lmax"])
+    f = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], use_fmm=True, leaf_size=32, fmm_d0=6.0, **kw)
+    d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
+    counts, levels = f.fmm_stats()
+    assert max(levels.values()) > 300
+    S, mask = _snm_flat(f.n_global, c["lmax"])
+    rng = np.random.default_rng(n)
+    xh = (rng.standard_normal(f.n) + 1j * rng.standard_normal(f.n)) * mask
+    x = xh / S
+    bf = f.calculate_vector_b(x) * S
+    bd = d.calculate_vector_b(x) * S
+    ta = np.linalg.norm(bd - xh) / np.linalg.norm(xh)
+    err = np.linalg.norm(bf - bd) / np.linalg.norm(bd - xh)
+    print(f"n={n} levels {levels}: |TA x|/|x| {ta:.3e}, MLFMM vs direct {err:.3e}")
+    assert 0.05 < ta < 5.0
+    assert err < 1e-7
+    f.close()
+    d.close()
