@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__re
 //   Block: 64 rows n x 64 translations; wave w owns rows 16 w .. 16 w + 15.
 constexpr int AM = 64;
 constexpr int AN = 64;
-constexpr int AK = 8;
+constexpr int AK = 16;
 
 __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict__ desc,
                                                      const double2 *__restrict__ wax, const double *__restrict__ wy,
@@ -422,15 +422,15 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             accP[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
             accQ[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
         }
-    double2 pa[2], pb[2];
-    double py[8];
+    double2 pa[4], pb[4];
+    double py[16];
     const bool ycol_ok = col0 + (tid >> 2) < NTOT;      // this thread's translation column
     const double *ybase = wy + cb + 4 * (int64_t)col0 + tid;
     auto load = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            int idx = tid + 256 * e;  // 64 rows x 8 inner
-            int r = idx >> 3, c = idx & 7;
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;  // 64 rows x 16 inner
+            int r = idx >> 4, c = idx & 15;
             int gr = row0 + r, gc = k0 + c;
             bool ok = gr < M && gc < KI;
             pa[e] = ok ? Pb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
         int nu = q + k0;
         int64_t row = rows_off(nu, K) + k + (nu < K ? nu : K);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < 16; ++e) {
             py[e] = (ycol_ok && nu <= Ls) ? ybase[row * NC] : 0.0;
             row += nu < K ? 2 * nu + 2 : 2 * K + 1;
             ++nu;
@@ -450,13 +450,13 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     load(0);
     for (int k0 = 0; k0 < KI; k0 += AK) {
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
+        for (int e = 0; e < 4; ++e) {
             int idx = tid + 256 * e;
-            Ps[idx & 7][idx >> 3] = pa[e];
-            Qs[idx & 7][idx >> 3] = pb[e];
+            Ps[idx & 15][idx >> 4] = pa[e];
+            Qs[idx & 15][idx >> 4] = pb[e];
         }
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < 16; ++e) {
             int idx = tid + 256 * e;
             Ys[idx >> 8][idx & 255] = py[e];
         }
