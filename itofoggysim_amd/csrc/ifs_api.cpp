@@ -168,6 +168,7 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         cur.maxK = std::max(cur.maxK, T.K);
         cur.max_nmax = std::max(cur.max_nmax, T.Lt + T.Ls + 2);
         cur.max_Ls = std::max(cur.max_Ls, T.Ls);
+        cur.max_Lt = std::max(cur.max_Lt, T.Lt);
         cur.max_rows_y = std::max(cur.max_rows_y, frow_size(T.Ls, T.K));
         cur.max_rows_z = std::max(cur.max_rows_z, rows_size(T.Lt, T.K));
         int nent = T.Lt * (T.Lt + 3) / 2;  // back-rotation entries (n, m >= 0)
@@ -240,6 +241,10 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
         int nsg = ch.s_end - ch.s_begin;
         if (nt <= 0) continue;
         const Task *tt = dt + ch.t_begin;
+        // one right-hand side and tables not kept: the coaxial coefficients are applied as
+        // they are generated and never stored (the table round trip through HBM is what
+        // the separate generate/apply pair spends its time on)
+        const bool fused = nrhs == 1 && !B.cached && ch.max_Lt < (ch.max_nmax + 2 <= 128 ? 256 : 1024);
         if (!(B.cached && B.tables_ready)) {
             {
                 Scope s(c, B.pn[0]);
@@ -249,7 +254,7 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
                 Scope s(c, B.pn[1]);
                 launch_gen_sect(tt, nt, ch.max_nmax, wsc, c->stream);
             }
-            {
+            if (!fused) {
                 Scope s(c, B.pn[2]);
                 launch_gen_axial(tt, nt, ch.maxK, ch.max_nmax, wsc, wax, c->stream);
             }
@@ -262,7 +267,10 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
                 Scope s(c, B.pn[3], ch.f_fwd);
                 launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, sp, wd, wy, c->stream);
             }
-            {
+            if (fused) {
+                Scope s(c, B.pn[4], ch.f_ax);
+                launch_axial_fused(tt, nt, ch.maxK, ch.max_nmax, wsc, wy, wz, c->stream);
+            } else {
                 Scope s(c, B.pn[4], ch.f_ax);
                 launch_axial_apply(tt, nt, ch.max_rows_z, wax, wy, wz, c->stream);
             }
@@ -318,6 +326,9 @@ int class_batch_finalize(ifs_ctx *c, ClassBatch &B) {
         const Task &x = B.gen[a], &y = B.gen[b];
         if (x.kind != y.kind) return x.kind < y.kind;
         if (x.t != y.t) return x.t < y.t;
+        // polar angles by |cos|: the runs of theta and pi - theta share one Wigner table
+        // and come out adjacent
+        if (std::fabs(x.cth) != std::fabs(y.cth)) return std::fabs(x.cth) < std::fabs(y.cth);
         if (x.cth != y.cth) return x.cth < y.cth;
         return x.phi < y.phi;
     });
@@ -596,9 +607,16 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     B.n_dslots = B.d_persist ? 0
                              : (int)std::max<size_t>(1, std::min(maxruns, (c->ws_budget / 4) / std::max<size_t>(dbytes, 1)));
     for (auto &ch : B.chunks) {
+        // consecutive runs with one Wigner key share a slot; a new key takes the next slot
+        int slot = -1, prev_key = -1;
         for (size_t r = 0; r < ch.runs.size(); ++r) {
             auto &P = ch.runs[r];
-            P.g.off_d = B.d_persist ? (int64_t)B.dkey[P.gen] * B.n_d : (int64_t)(r % B.n_dslots) * B.n_d;
+            const int key = B.dkey[P.gen];
+            if (key != prev_key) {
+                slot = B.n_dslots > 0 ? (slot + 1) % B.n_dslots : 0;
+                prev_key = key;
+            }
+            P.g.off_d = B.d_persist ? (int64_t)key * B.n_d : (int64_t)slot * B.n_d;
             P.g.mirror = B.gen[P.gen].cth < 0.0 ? 1 : 0;
             B.descs[P.di] = P.g;
         }

@@ -42,7 +42,7 @@ struct Chunk {
     int t_begin = 0, t_end = 0;      // task range (device task array)
     int s_begin = 0, s_end = 0;      // segment range
     int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;
-    int maxpairs = 0, maxK = 0, max_nmax = 0, max_Ls = 0;
+    int maxpairs = 0, maxK = 0, max_nmax = 0, max_Ls = 0, max_Lt = 0;
     int64_t max_rows_y = 0, max_rows_z = 0;
     int rowback_blocks = 1;
     double f_fwd = 0.0, f_ax = 0.0, f_back = 0.0;  // algorithmic flops of the chunk's products

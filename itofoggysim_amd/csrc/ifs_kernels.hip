@@ -350,6 +350,202 @@ __global__ void k_gen_axial(const Task *__restrict__ tasks, int max_nmax, const 
     }
 }
 
+// ------------------------------------------------------------------------
+// Coaxial shift with its coefficients generated on the fly (one right-hand side):
+// the recurrence of k_gen_axial, and instead of storing P_{+-m}, Q_{+-m} each
+// coefficient is applied at once,
+//   z_+(n, m) += P_m y_+(nu, m),   z_-(n, m) += Q_m y_-(nu, m),
+//   z_+(n,-m) += Q_m y_+(nu,-m),   z_-(n,-m) += P_m y_-(nu,-m)   (P_{-m} = Q_m, Q_{-m} = P_m),
+// so the tables never reach HBM.  Block per (order m >= 0, task) of 64 or 256
+// threads; rows of a thread: FR at most (target orders up to FR * threads - 1).
+// ------------------------------------------------------------------------
+constexpr int FR = 4;
+__global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, const double2 *__restrict__ wsect,
+                              const double2 *__restrict__ wy, double2 *__restrict__ wz) {
+    extern __shared__ double2 sh[];
+    const Task T = tasks[blockIdx.y];
+    const int m = blockIdx.x;
+    if (m > T.K) return;
+    const int nmax = T.Lt + T.Ls + 2;
+    const int Lt = T.Lt, Ls = T.Ls, K = T.K;
+    const double t = T.t;
+    double2 *P = sh;
+    double2 *C = sh + (max_nmax + 2);
+    double2 *N = sh + 2 * (max_nmax + 2);
+    double2 *red = sh + 3 * (max_nmax + 2);  // 4 waves x 4 partial sums
+    const double2 *src = wsect + T.off_s + (int64_t)m * (nmax + 2);
+    int top = nmax - m;
+    for (int n = threadIdx.x; n <= nmax + 1; n += blockDim.x) {
+        P[n] = make_double2(0.0, 0.0);
+        C[n] = (n >= m && n <= top) ? src[n] : make_double2(0.0, 0.0);
+        N[n] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int q = m < 1 ? 1 : m;
+    const int64_t ny = rows_size(Ls, K), nz = rows_size(Lt, K);
+    const double2 *yp = wy + T.off_y;  // y_+ rows (nu, k); y_- at + ny
+    double2 *zp = wz + T.off_z;        // z_+ rows (n, k); z_- at + nz
+    auto yrow = [&](int nu, int k) { return rows_off(nu, K) + k + (nu < K ? nu : K); };
+    // P = A + B, Q = A - B of target order n, source order nu
+    auto coef = [&](int n, int nu, double2 c0, double2 cp, double2 cm, double2 &pv, double2 &qv) {
+        const double nn = (double)n;
+        const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1));
+        double2 s = cscale(sqrt(nn * (nn + 1.0)), c0);
+        s = cadd(s, cscale(t * coef_a(n, m) * sqrt(nn / (nn + 1.0)), cp));
+        s = cadd(s, cscale(t * coef_a(n - 1, m) * sqrt((nn + 1.0) / nn), cm));
+        double2 av = ipow_mul((nu - n) & 3, cscale(inv_nu, s));
+        double2 bv = ipow_mul((nu - n + 1) & 3, cscale(t * (double)m * inv_nu / sqrt(nn * (nn + 1.0)), c0));
+        pv = cadd(av, bv);
+        qv = csub(av, bv);
+    };
+    if (Ls > Lt) {
+        // recurrence over the target order (see k_gen_axial); the sum over nu is a
+        // block reduction per target row
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        for (int c = m;; ++c) {
+            if (c <= Lt) {
+                const double ac = coef_a(c, m);
+                const double acm1 = coef_a(c - 1, m);
+                for (int r = m + threadIdx.x; r <= top - 1; r += blockDim.x) {
+                    double2 v = cscale(acm1, P[r]);
+                    v = csub(v, cscale(coef_a(r, m), C[r + 1]));
+                    if (r - 1 >= m) v = cadd(v, cscale(coef_a(r - 1, m), C[r - 1]));
+                    N[r] = cscale(1.0 / ac, v);
+                }
+                for (int r = top + threadIdx.x; r <= nmax + 1; r += blockDim.x) N[r] = make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+            if (c >= q && c <= Lt) {
+                const int n = c;
+                double2 s4[4] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0),
+                                 make_double2(0.0, 0.0)};
+                for (int nu = q + threadIdx.x; nu <= Ls; nu += blockDim.x) {
+                    const double sg = ((n + nu) & 1) ? -1.0 : 1.0;
+                    double2 c0 = cscale(sg, C[nu]);
+                    double2 cp = cscale(-sg, N[nu]);
+                    double2 cm = (n - 1 >= m) ? cscale(-sg, P[nu]) : make_double2(0.0, 0.0);
+                    double2 pv, qv;
+                    coef(n, nu, c0, cp, cm, pv, qv);
+                    const int64_t iy = yrow(nu, m);
+                    s4[0] = cadd(s4[0], cmul(pv, yp[iy]));
+                    s4[1] = cadd(s4[1], cmul(qv, yp[iy + ny]));
+                    if (m > 0) {
+                        const int64_t iyn = yrow(nu, -m);
+                        s4[2] = cadd(s4[2], cmul(qv, yp[iyn]));
+                        s4[3] = cadd(s4[3], cmul(pv, yp[iyn + ny]));
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) {
+                        s4[i].x += __shfl_xor(s4[i].x, o, 64);
+                        s4[i].y += __shfl_xor(s4[i].y, o, 64);
+                    }
+                if (lane == 0)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) red[4 * w + i] = s4[i];
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    double2 tot[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        tot[i] = red[i];
+                        for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww) tot[i] = cadd(tot[i], red[4 * ww + i]);
+                    }
+                    const int64_t iz = rows_off(n, K) + m + (n < K ? n : K);
+                    zp[iz] = tot[0];
+                    zp[iz + nz] = tot[1];
+                    if (m > 0) {
+                        const int64_t izn = rows_off(n, K) - m + (n < K ? n : K);
+                        zp[izn] = tot[2];
+                        zp[izn + nz] = tot[3];
+                    }
+                }
+            }
+            if (c >= Lt) break;
+            __syncthreads();
+            double2 *tmp = P;
+            P = C;
+            C = N;
+            N = tmp;
+            top -= 1;
+        }
+        return;
+    }
+    double2 acc[FR][4];
+#pragma unroll
+    for (int j = 0; j < FR; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[j][i] = make_double2(0.0, 0.0);
+    for (int nu = m; nu <= Ls; ++nu) {
+        if (nu >= 1) {
+            const int64_t iy = yrow(nu, m);
+            const double2 ypp = yp[iy], ypm = yp[iy + ny];
+            double2 ynp = make_double2(0.0, 0.0), ynm = ynp;
+            if (m > 0) {
+                const int64_t iyn = yrow(nu, -m);
+                ynp = yp[iyn];
+                ynm = yp[iyn + ny];
+            }
+#pragma unroll
+            for (int j = 0; j < FR; ++j) {
+                const int n = q + threadIdx.x + j * blockDim.x;
+                if (n > Lt) break;
+                double2 c0 = C[n];
+                double2 cp = C[n + 1];
+                double2 cm = (n - 1 >= m) ? C[n - 1] : make_double2(0.0, 0.0);
+                double2 pv, qv;
+                coef(n, nu, c0, cp, cm, pv, qv);
+                acc[j][0] = cadd(acc[j][0], cmul(pv, ypp));
+                acc[j][1] = cadd(acc[j][1], cmul(qv, ypm));
+                acc[j][2] = cadd(acc[j][2], cmul(qv, ynp));
+                acc[j][3] = cadd(acc[j][3], cmul(pv, ynm));
+            }
+        }
+        if (nu == Ls) break;
+        const double anu = coef_a(nu, m);
+        const double anum1 = coef_a(nu - 1, m);
+        for (int n = m + threadIdx.x; n <= top - 1; n += blockDim.x) {
+            double2 v = cscale(anum1, P[n]);
+            v = csub(v, cscale(coef_a(n, m), C[n + 1]));
+            if (n - 1 >= m) v = cadd(v, cscale(coef_a(n - 1, m), C[n - 1]));
+            N[n] = cscale(1.0 / anu, v);
+        }
+        __syncthreads();
+        double2 *tmp = P;
+        P = C;
+        C = N;
+        N = tmp;
+        top -= 1;
+        for (int n = top + 1 + threadIdx.x; n <= nmax + 1; n += blockDim.x) C[n] = make_double2(0.0, 0.0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < FR; ++j) {
+        const int n = q + threadIdx.x + j * blockDim.x;
+        if (n > Lt) break;
+        const int kk = n < K ? n : K;
+        const int64_t iz = rows_off(n, K) + m + kk;
+        zp[iz] = acc[j][0];
+        zp[iz + nz] = acc[j][1];
+        if (m > 0) {
+            const int64_t izn = rows_off(n, K) - m + kk;
+            zp[izn] = acc[j][2];
+            zp[izn + nz] = acc[j][3];
+        }
+    }
+}
+
+void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect,
+                        const double2 *wy, double2 *wz, hipStream_t s) {
+    if (ntasks <= 0) return;
+    size_t shm = sizeof(double2) * (3 * (size_t)(max_nmax + 2) + 16);
+    // sphere-sphere orders (<= ~60 each) fill one wave; box orders take four
+    const int threads = max_nmax + 2 <= 128 ? 64 : 256;
+    hipLaunchKernelGGL(k_axial_fused, dim3(maxK + 1, ntasks), dim3(threads), shm, s, tasks, max_nmax, wsect, wy, wz);
+}
+
 void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect, double2 *wax,
                       hipStream_t s) {
     if (ntasks <= 0) return;

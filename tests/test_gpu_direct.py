@@ -29,13 +29,16 @@ def _small_cluster(ns, lmax_hi, seed):
     return pos, radii, lmax
 
 
+@pytest.mark.parametrize("cache", [0, -1])
 @pytest.mark.parametrize("conv", ["physical", "reference"])
 @pytest.mark.parametrize("zsel", [3, 1])
 @pytest.mark.parametrize("ns,lhi,seed", [(2, 3, 0), (5, 6, 1), (7, 9, 2)])
-def test_direct_matvec_small(oracle, conv, zsel, ns, lhi, seed):
+def test_direct_matvec_small(oracle, conv, zsel, ns, lhi, seed, cache):
+    """cache 0: tables generated once and kept; cache -1: coaxial coefficients generated
+    and applied on the fly in every matvec (per-pair orders Ls > Lt and Ls <= Lt both occur)."""
     pos, radii, lmax = _small_cluster(ns, lhi, seed)
     k = 1.3
-    s = MultiSphereSolver(pos, radii, 1.5 + 0.01j, k, lmax=lmax, z_selector=zsel, d_convention=conv)
+    s = MultiSphereSolver(pos, radii, 1.5 + 0.01j, k, lmax=lmax, z_selector=zsel, d_convention=conv, cache_mb=cache)
     an, bn = s.mie_coefficients()
     x = _rand(s.n, seed + 10)
     b = s.calculate_vector_b(x)
@@ -58,10 +61,11 @@ def test_mie_vs_oracle(oracle):
     s.close()
 
 
-def test_direct_matvec_config1(oracle):
+@pytest.mark.parametrize("cache", [0, -1])
+def test_direct_matvec_config1(oracle, cache):
     """20 water spheres, U[0.3,1] um in a 10 um cube, lambda 550 nm (L_j 14..23)."""
     c = config_cluster(1)
-    s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"])
+    s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"], cache_mb=cache)
     an, bn = s.mie_coefficients()
     x = _rand(s.n, 7)
     b = s.calculate_vector_b(x)

@@ -80,9 +80,10 @@ def test_fmm_chunked_workspace_config1():
 
 
 def test_fmm_table_cache_modes_config1():
-    """Tables kept across matvecs vs regenerated per matvec in workspace slots, with the
-    classes cut into many chunks: the same products, so the same result to rounding of
-    the summation order (none here: identical kernels and order)."""
+    """Tables kept across matvecs vs regenerated per matvec in workspace slots (class
+    tables) or generated and applied on the fly (sphere-level coaxial coefficients), with
+    the classes cut into many chunks: the same products, summed in another order only
+    where the on-the-fly shift reduces over source orders (box to sphere)."""
     c = config_cluster(1)
     kw = dict(lmax=c["lmax"], use_fmm=True, leaf_size=2, fmm_d0=6.0)
     a = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
@@ -94,9 +95,10 @@ def test_fmm_table_cache_modes_config1():
     yd = d.calculate_vector_b(x)
     assert np.array_equal(ya, a.calculate_vector_b(x))
     assert np.array_equal(yb, b.calculate_vector_b(x))
-    assert np.array_equal(ya, yb)
+    e1 = np.linalg.norm(ya - yb) / np.linalg.norm(ya - x)
     err = np.linalg.norm(ya - yd) / np.linalg.norm(ya - x)
-    print("kept vs regenerated tables, chunked", err)
+    print("kept vs regenerated tables", e1, "chunked", err)
+    assert e1 < 1e-13
     assert err < 1e-13
     for s_ in (a, b, d):
         s_.close()
