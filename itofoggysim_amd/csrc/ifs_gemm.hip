@@ -406,7 +406,10 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     // 8 consecutive lanes, which a 1 KiB row stride would put on the same banks
     __shared__ double2 Ps[AK][AM + 1];
     __shared__ double2 Qs[AK][AM + 1];
-    __shared__ double Ys[AK][4 * AN + 2];
+    // Y tile by part: the 16 lanes of an MFMA B fragment read 16 consecutive columns of
+    // one part; a part row of AN + 16 doubles puts the two parts one half-wave reads
+    // 32 banks apart (no bank conflicts on the fragment loads)
+    __shared__ double Ys[AK][4][AN + 16];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -456,10 +459,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             Qs[idx & 15][idx >> 4] = pb[e];
         }
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            int idx = tid + 256 * e;
-            Ys[idx >> 8][idx & 255] = py[e];
-        }
+        for (int e = 0; e < 16; ++e) Ys[e][tid & 3][tid >> 2] = py[e];
         __syncthreads();
         if (k0 + AK < KI) load(k0 + AK);
 #pragma unroll
@@ -475,8 +475,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int col = 16 * j + li;
-                bP[j] = Ys[cc + nul][4 * col + pin];
-                bQ[j] = Ys[cc + nul][4 * col + 2 + pin];
+                bP[j] = Ys[cc + nul][pin][col];
+                bQ[j] = Ys[cc + nul][2 + pin][col];
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
