@@ -145,7 +145,7 @@ int ifs_amplitude_matrix(int32_t ndir, const double *phi, const double *Fx, cons
 int ifs_mueller(int32_t ndir, const double *S, double *M);
 
 typedef struct {
-    int32_t restart;      /* Krylov subspace size (reference default 10; BASELINE uses 50) */
+    int32_t restart;      /* Krylov subspace size (reference default 10; the 2000-sphere benchmark uses 50) */
     int32_t max_iter;     /* maximum matvecs */
     double tol;           /* relative residual ||b - M x|| / ||b|| */
     int32_t use_x0;       /* 1: x holds the initial guess on entry */
