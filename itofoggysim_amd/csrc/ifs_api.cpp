@@ -265,7 +265,10 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
             double2 *dp = dst_pool + r * dst_stride;
             {
                 Scope s(c, B.pn[3], ch.f_fwd);
-                launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, sp, wd, wy, c->stream);
+                if (ch.max_Ls >= 128)
+                    launch_rot_fwd_wide(tt, nt, ch.max_Ls, sp, wd, wy, c->stream);
+                else
+                    launch_rot_fwd(tt, nt, ch.max_rows_y, ch.max_Ls, sp, wd, wy, c->stream);
             }
             if (fused) {
                 Scope s(c, B.pn[4], ch.f_ax);

@@ -157,6 +157,9 @@ void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, con
 // max_rows: largest frow_size(Ls, K) of the tasks
 void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls, const double2 *src_pool,
                     const double *wd, double2 *wy, hipStream_t s);
+// Forward rotation for high source orders: block per (task, degree), lanes along m.
+void launch_rot_fwd_wide(const Task *tasks, int ntasks, int max_Ls, const double2 *src_pool, const double *wd,
+                         double2 *wy, hipStream_t s);
 void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const double2 *wax, const double2 *wy,
                         double2 *wz, hipStream_t s);
 // Coaxial shift Y -> Z with the coefficients generated on the fly (needs the

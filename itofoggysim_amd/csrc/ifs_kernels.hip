@@ -627,6 +627,89 @@ __global__ void k_rot_fwd(const Task *__restrict__ tasks, const double2 *__restr
     }
 }
 
+// The same forward rotation for high source orders (box to sphere): one block per
+// (task, degree nu).  The folded column entries s_m, t_m of both parts go to LDS once;
+// each wave then takes rows k and reads its E and O rows with the lanes along m
+// (coalesced), reducing over m with shuffles.
+__global__ void k_rot_fwd_wide(const Task *__restrict__ tasks, const double2 *__restrict__ src_pool,
+                               const double *__restrict__ wd, double2 *__restrict__ wy) {
+    extern __shared__ double2 stm[];
+    const Task T = tasks[blockIdx.x];
+    const int nu = blockIdx.y + 1;
+    if (nu > T.Ls) return;
+    const int Ls = T.Ls, K = T.K;
+    const int kk = nu < K ? nu : K;
+    const int W = nu + 1;
+    double2 *sp = stm, *sm = stm + W, *tp = stm + 2 * W, *tm = stm + 3 * W;
+    const double2 *xa = src_pool + T.src + fidx(nu, 0);  // xa[m], m = -nu..nu
+    const double2 *xb = xa + (int64_t)Ls * (Ls + 2);
+    for (int m = threadIdx.x; m <= nu; m += blockDim.x) {
+        double sn, cs;
+        sincos((double)m * T.phi, &sn, &cs);
+        const double2 p = make_double2(cs, sn);
+        const double2 ap = xa[m], bp = xb[m];
+        const double2 up = cmul(p, cadd(ap, bp)), um = cmul(p, csub(ap, bp));
+        if (m == 0) {
+            sp[0] = up;
+            sm[0] = um;
+            tp[0] = tm[0] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const double2 pc = make_double2(p.x, -p.y);
+        const double2 an = xa[-m], bn = xb[-m];
+        const double2 vp = cmul(pc, cadd(an, bn)), vm = cmul(pc, csub(an, bn));
+        const double sg = (m & 1) ? -1.0 : 1.0;
+        sp[m] = cadd(up, cscale(sg, vp));
+        sm[m] = cadd(um, cscale(sg, vm));
+        tp[m] = csub(up, cscale(sg, vp));
+        tm[m] = csub(um, cscale(sg, vm));
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int64_t nent = rows_size(Ls, K);
+    double2 *y = wy + T.off_y;
+    for (int k = threadIdx.x >> 6; k <= kk; k += nw) {
+        const double *E = wd + T.off_d + fold_off(nu, K) + (int64_t)k * W;
+        const double *O = E + (int64_t)(kk + 1) * W;
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // S+, S-, T+, T- (re, im)
+        for (int m = lane; m <= nu; m += 64) {
+            const double e = E[m], o = O[m];
+            a[0] += e * sp[m].x;
+            a[1] += e * sp[m].y;
+            a[2] += e * sm[m].x;
+            a[3] += e * sm[m].y;
+            a[4] += o * tp[m].x;
+            a[5] += o * tp[m].y;
+            a[6] += o * tm[m].x;
+            a[7] += o * tm[m].y;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) a[i] += __shfl_xor(a[i], off, 64);
+        if (lane == 0) {
+            const double2 Sp = make_double2(a[0], a[1]), Sm = make_double2(a[2], a[3]);
+            const double2 Tp = make_double2(a[4], a[5]), Tm = make_double2(a[6], a[7]);
+            const int64_t rp = rows_off(nu, K) + kk + k;
+            y[rp] = cscale(0.5, cadd(Sp, Tp));
+            y[rp + nent] = cscale(0.5, cadd(Sm, Tm));
+            if (k > 0) {
+                const double f = (k & 1) ? -0.5 : 0.5;
+                const int64_t rm = rows_off(nu, K) + kk - k;
+                y[rm] = cscale(f, csub(Sp, Tp));
+                y[rm + nent] = cscale(f, csub(Sm, Tm));
+            }
+        }
+    }
+}
+
+void launch_rot_fwd_wide(const Task *tasks, int ntasks, int max_Ls, const double2 *src_pool, const double *wd,
+                         double2 *wy, hipStream_t s) {
+    if (ntasks <= 0 || max_Ls <= 0) return;
+    size_t shm = sizeof(double2) * 4 * (size_t)(max_Ls + 1);
+    hipLaunchKernelGGL(k_rot_fwd_wide, dim3(ntasks, max_Ls), dim3(256), shm, s, tasks, src_pool, wd, wy);
+}
+
 void launch_rot_fwd(const Task *tasks, int ntasks, int64_t max_rows, int max_Ls, const double2 *src_pool,
                     const double *wd, double2 *wy, hipStream_t s) {
     if (ntasks <= 0) return;
