@@ -115,15 +115,18 @@ def _snm_flat(N, lmax):
     return np.tile(np.concatenate([s, s]), len(lmax)), mask
 
 
-@pytest.mark.parametrize("n", [200, 600])
-def test_fmm_vs_direct_high_order(n):
+@pytest.mark.parametrize("n,cache", [(200, 0), (600, 0), (200, -1)])
+def test_fmm_vs_direct_high_order(n, cache):
     """Clusters of the 2000-sphere generator with n spheres: one MLFMM level with box
     order 377 (n = 200) or 536 (n = 600), far beyond the orders where the coaxial
     recurrences hold.  The MLFMM matvec must match the direct one, compared in the
-    normalised basis (the flat basis spans 1e+-80 at N = 49)."""
+    normalised basis (the flat basis spans 1e+-80 at N = 49).  cache -1: nothing kept,
+    so every table is regenerated in the matvec and the box-to-sphere and sphere-pair
+    coaxial shifts generate their coefficients on the fly."""
     c = config_cluster(2, n=n)
     kw = dict(lmax=c["lmax"])
-    f = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], use_fmm=True, leaf_size=32, fmm_d0=6.0, **kw)
+    f = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], use_fmm=True, leaf_size=32, fmm_d0=6.0,
+                          cache_mb=cache, **kw)
     d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
     counts, levels = f.fmm_stats()
     assert max(levels.values()) > 300
