@@ -53,14 +53,19 @@ constexpr int RPAD = 2;  // LDS row padding (doubles)
 //   E(pi - b) = sigma E(b),  O(pi - b) = -sigma O(b),
 // so the column entries take (-1)^m (forward) or (-1)^k (back) at the LDS store and
 // the results (-1)^{n+k} (forward) or (-1)^{n+m} (back), with T and G negated.
-// Back product block: 64 rows of each product x 128 real columns; wave w owns
-// rows 32 (w / 2) .. + 31 and columns 64 (w % 2) .. + 63 of both products, so F
-// and G of one entry meet in the same lane.  Forward product block: 32 rows of
-// each product x 128 columns; waves 0,1 compute E, waves 2,3 O (64 columns each),
-// and S and T meet through LDS in the epilogue (at 64 rows the forward kernel's
-// gather registers push it to one wave per SIMD, which is slower).
+// Back product block: 64 rows of each product x FNB = 64 real columns; wave w owns
+// rows 32 (w / 2) .. + 31 and columns 32 (w % 2) .. + 31 of both products, so F
+// and G of one entry meet in the same lane; registers bounded for three waves per
+// SIMD.  Forward product block: 32 rows of each product x FNF2 = 64 columns; waves
+// 0,1 compute E, waves 2,3 O (32 columns each), and S and T meet through LDS in the
+// epilogue (at 64 rows the forward kernel's gather registers push it to one wave
+// per SIMD, which is slower).  The narrow column tiles buy occupancy: a tile's
+// gather is issued one inner step ahead, and three waves per SIMD hide its
+// latency better than two.
 constexpr int FM = 64;
 constexpr int FMF = 32;
+constexpr int FNF2 = 64;
+constexpr int FNB = 64;
 constexpr int FN = 128;
 constexpr int FK = 16;
 
@@ -81,28 +86,28 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     const int NC = 4 * NTOT;
     const int NCY = 4 * C.ncy * C.nrhs;
     const int row0 = blockIdx.y * FMF;
-    const int col0 = blockIdx.x * FN;
+    const int col0 = blockIdx.x * FNF2;
     if (row0 >= M) return;
     __shared__ double As[FK][2 * FMF + RPAD];     // columns 0..31: E rows, 32..63: O rows
-    __shared__ double Bs[2][FK][FN + RPAD];      // s, t
+    __shared__ double Bs[2][FK][FNF2 + RPAD];      // s, t
     const double *E = wd + C.off_d + fold_off(nu, K);
     const double *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int g = w >> 1;            // 0: E product, 1: O product
-    const int wc = (w & 1) * 64;
-    dv4 acc[2][4];
+    const int wc = (w & 1) * 32;
+    dv4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 2; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
     const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
     // gather: thread -> (translation t, inner q), idx = tid + 256 e, t = idx / 16, q = idx % 16
     const double2 *xs[2];
     const double2 *ps[2];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
+    for (int e = 0; e < 1; ++e) {
         int t = (tid + 256 * e) >> 4;
         int j = (col0 >> 2) + t;
         bool ok = j < NTOT;
@@ -122,7 +127,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             ra[e] = (gr < M && gq < KI) ? src[(int64_t)gr * KI + gq] : 0.0;
         }
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
+        for (int e = 0; e < 1; ++e) {
             int gq = k0 + ((tid + 256 * e) & 15);
             if (gq < KI && xs[e]) {
                 rap[e] = xs[e][gq];
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             As[idx & 15][idx >> 4] = ra[e];
         }
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
+        for (int e = 0; e < 1; ++e) {
             int idx = tid + 256 * e;
             int t = idx >> 4, q = idx & 15;
             int m = k0 + q;
@@ -179,27 +184,27 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         if (k0 + FK < KI) load(k0 + FK);
 #pragma unroll
         for (int kc = 0; kc < FK; kc += 4) {
-            double a[2], b[4];
+            double a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
+            for (int j = 0; j < 2; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
     // epilogue: the O-product waves hand T to the E-product waves through LDS
-    double *X = &Bs[0][0][0];  // 32 x (FN + RPAD) doubles
+    double *X = &Bs[0][0][0];  // 32 x (FNF2 + RPAD) doubles
     if (g == 1) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FN + RPAD) + wc + 16 * j + li] = acc[i][j][v];
+                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FNF2 + RPAD) + wc + 16 * j + li] = acc[i][j][v];
     }
     __syncthreads();
     if (g == 1) return;
@@ -207,7 +212,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 2; ++j) {
             int c = wc + 16 * j + li;
             int gc = col0 + c;
             if (gc >= NC) continue;
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
                 int r = 16 * i + 4 * v + lk;
                 int k = row0 + r;
                 if (k >= M) continue;
-                double S = acc[i][j][v], T = X[r * (FN + RPAD) + c];
+                double S = acc[i][j][v], T = X[r * (FNF2 + RPAD) + c];
                 if (C.mirror) {
                     double f = ((nu + k) & 1) ? -1.0 : 1.0;
                     S *= f;
@@ -228,7 +233,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         }
 }
 
-__global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__restrict__ desc,
+__global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__restrict__ desc,
                                                         const int32_t *__restrict__ pidx,
                                                         const double2 *__restrict__ phase,
                                                         const double *__restrict__ wd, const double *__restrict__ wz,
@@ -244,10 +249,10 @@ __global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__re
     const int NC = 4 * NTOT;
     const int NCY = 4 * C.ncy * C.nrhs;
     const int row0 = blockIdx.y * FM;
-    const int col0 = blockIdx.x * FN;
+    const int col0 = blockIdx.x * FNB;
     if (row0 >= M) return;
     __shared__ double As[FK][2 * FM + RPAD];     // columns 0..FM-1: E^T rows, FM..2FM-1: O^T rows
-    __shared__ double Bs[2][FK][FN + RPAD];      // s, t
+    __shared__ double Bs[2][FK][FNB + RPAD];     // s, t
     const double *E = wd + C.off_d + fold_off(n, K);  // E[k][m], KI x M
     const double *O = E + (int64_t)KI * M;
     const double *Z = wz + rows_off(n, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;  // row kk + k
@@ -255,18 +260,18 @@ __global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__re
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int wr = (w >> 1) * 32;
-    const int wc = (w & 1) * 64;
-    dv4 accF[2][4], accG[2][4];
+    const int wc = (w & 1) * (FNB / 2);
+    dv4 accF[2][FNB / 32], accG[2][FNB / 32];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FNB / 32; ++j) {
             accF[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
             accG[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
         }
     // raw loads only: the s/t combination happens at the LDS store, so that the
     // prefetch of the next step stays in flight while the matrix cores work
-    double ra[8], rzp[8], rzm[8];
+    double ra[8], rzp[FNB / 16], rzm[FNB / 16];
     auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -277,9 +282,9 @@ __global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__re
             ra[e] = (gm < M && gk < KI) ? src[(int64_t)gk * M + gm] : 0.0;
         }
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            int idx = tid + 256 * e;  // 16 inner x 128 columns
-            int q = idx >> 7, c = idx & 127;
+        for (int e = 0; e < FNB / 16; ++e) {
+            int idx = tid + 256 * e;  // 16 inner x FNB columns
+            int q = idx / FNB, c = idx % FNB;
             int gk = k0 + q, gc = col0 + c;
             bool ok = gk < KI && gc < NC;
             rzp[e] = ok ? Z[(int64_t)(kk + gk) * NCY + gc] : 0.0;
@@ -294,35 +299,35 @@ __global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__re
             As[idx >> 7][idx & 127] = ra[e];
         }
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < FNB / 16; ++e) {
             int idx = tid + 256 * e;
-            int q = idx >> 7;
+            int q = idx / FNB, c = idx % FNB;
             int gk = k0 + q;
             // s_k = z_k + (-1)^k z_{-k}, t_k = z_k - (-1)^k z_{-k}; s_0 = z_0, t_0 = 0 (rzm = 0 at k = 0)
             double sg = (gk & 1) ? -1.0 : 1.0;
             double f = (C.mirror && (gk & 1)) ? -1.0 : 1.0;  // (-1)^k on s_k and t_k
-            Bs[0][q][idx & 127] = f * (rzp[e] + sg * rzm[e]);
-            Bs[1][q][idx & 127] = f * (rzp[e] - sg * rzm[e]);
+            Bs[0][q][c] = f * (rzp[e] + sg * rzm[e]);
+            Bs[1][q][c] = f * (rzp[e] - sg * rzm[e]);
         }
         __syncthreads();
         if (k0 + FK < KI) load(k0 + FK);
 #pragma unroll
         for (int kc = 0; kc < FK; kc += 4) {
-            double ae[2], ao[2], bs[4], bt[4];
+            double ae[2], ao[2], bs[FNB / 32], bt[FNB / 32];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 ae[i] = As[kc + lk][wr + 16 * i + li];
                 ao[i] = As[kc + lk][FM + wr + 16 * i + li];
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < FNB / 32; ++j) {
                 bs[j] = Bs[0][kc + lk][wc + 16 * j + li];
                 bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < FNB / 32; ++j) {
                     accF[i][j] = mfma(ae[i], bs[j], accF[i][j]);
                     accG[i][j] = mfma(ao[i], bt[j], accG[i][j]);
                 }
@@ -334,7 +339,7 @@ __global__ __launch_bounds__(256, 2) void k_class_rot_back(const GroupDesc *__re
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < FNB / 32; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 int c = wc + 16 * j + li;
@@ -538,7 +543,7 @@ void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *sr
     if (h.ncols <= 0 || h.nrhs <= 0) return;
     if (stage == 0) {
         int mt = ((h.Ls < K ? h.Ls : K) + 1 + FMF - 1) / FMF;
-        dim3 grid((NC + FN - 1) / FN, mt, h.Ls);
+        dim3 grid((NC + FNF2 - 1) / FNF2, mt, h.Ls);
         hipLaunchKernelGGL(k_class_rot_fwd, grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
@@ -546,7 +551,7 @@ void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *sr
         hipLaunchKernelGGL(k_class_axial, grid, dim3(256), 0, s, d_desc, wax, wy, wz);
     } else {
         int mt = (h.Lt + 1 + FM - 1) / FM;
-        dim3 grid((NC + FN - 1) / FN, mt, h.Lt);
+        dim3 grid((NC + FNB - 1) / FNB, mt, h.Lt);
         hipLaunchKernelGGL(k_class_rot_back, grid, dim3(256), 0, s, d_desc, pidx, phase, wd, wz, (double2 *)wy);
     }
 }
