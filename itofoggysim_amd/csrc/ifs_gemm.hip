@@ -66,7 +66,6 @@ constexpr int FM = 64;
 constexpr int FMF = 32;
 constexpr int FNF2 = 64;
 constexpr int FNB = 64;
-constexpr int FN = 128;
 constexpr int FK = 16;
 
 __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restrict__ desc,
