@@ -81,9 +81,20 @@ __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__rest
     const double sh = sqrt(fmax(0.0, 0.5 * (1.0 - cb)));
     const double sgm = (m & 1) ? -1.0 : 1.0;
     double *base = wd + T.off_d;
+    // degree n's E block starts at fold_off(n, K); it advances by 2 (min(n,K)+1)(n+1)
+    // per degree, so the offsets are carried instead of evaluated per degree
+    int64_t fo = -1;
+    int fn = -1;
     auto put = [&](int n, double dp, double dm) {
-        int kk = n < K ? n : K;
-        int64_t o = fold_off(n, K) + (int64_t)k * (n + 1) + m;
+        const int kk = n < K ? n : K;
+        if (fn < 0) {
+            fo = fold_off(n, K);
+        } else {
+            const int kp = fn < K ? fn : K;
+            fo += 2 * (int64_t)(kp + 1) * (fn + 1);
+        }
+        fn = n;
+        const int64_t o = fo + (int64_t)k * (n + 1) + m;
         double e, od;
         if (m == 0) {
             e = 2.0 * dp;
