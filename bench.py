@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-pairs", type=int, default=160)
+    ap.add_argument("--cpu-pairs", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES(50) solve to 1e-6")
     ap.add_argument("--solve-max-iter", type=int, default=100)
