@@ -21,6 +21,8 @@
 // no atomics).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ifs_internal.h"
 
 namespace ifs {
@@ -140,6 +142,33 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             }
         }
     };
+    // MFMA tiles wholly in the padding of the last row, column or inner tile are
+    // skipped per wave: live 16-row tiles and 16-column tiles of this wave
+    const int ni = M - row0 >= 32 ? 2 : (M - row0 + 15) / 16;
+    const int ncw = NC - col0 - wc;
+    const int nj = ncw >= 32 ? 2 : (ncw <= 0 ? 0 : (ncw + 15) / 16);
+    auto step = [&](int k0, auto full) {
+#pragma unroll
+        for (int kc = 0; kc < FK; kc += 4) {
+            if constexpr (!decltype(full)::value) {
+                if (k0 + kc >= KI) break;
+            }
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if constexpr (!decltype(full)::value) {
+                        if (i >= ni || j >= nj) continue;
+                    }
+                    acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                }
+        }
+    };
     load(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
@@ -181,18 +210,10 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         }
         __syncthreads();
         if (k0 + FK < KI) load(k0 + FK);
-#pragma unroll
-        for (int kc = 0; kc < FK; kc += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-        }
+        if (ni == 2 && nj == 2 && k0 + FK <= KI)
+            step(k0, std::true_type{});
+        else if (ni > 0 && nj > 0)
+            step(k0, std::false_type{});
         __syncthreads();
     }
     // epilogue: the O-product waves hand T to the E-product waves through LDS
@@ -290,6 +311,41 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
             rzm[e] = (ok && gk > 0) ? Z[(int64_t)(kk - gk) * NCY + gc] : 0.0;
         }
     };
+    // MFMA tiles wholly in the padding of the last row, column or inner tile are
+    // skipped per wave: live 16-row tiles and 16-column tiles of this wave
+    const int nrw = M - row0 - wr;
+    const int ni = nrw >= 32 ? 2 : (nrw <= 0 ? 0 : (nrw + 15) / 16);
+    const int ncw = NC - col0 - wc;
+    const int nj = ncw >= FNB / 2 ? FNB / 32 : (ncw <= 0 ? 0 : (ncw + 15) / 16);
+    auto step = [&](int k0, auto full) {
+#pragma unroll
+        for (int kc = 0; kc < FK; kc += 4) {
+            if constexpr (!decltype(full)::value) {
+                if (k0 + kc >= KI) break;
+            }
+            double ae[2], ao[2], bs[FNB / 32], bt[FNB / 32];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ae[i] = As[kc + lk][wr + 16 * i + li];
+                ao[i] = As[kc + lk][FM + wr + 16 * i + li];
+            }
+#pragma unroll
+            for (int j = 0; j < FNB / 32; ++j) {
+                bs[j] = Bs[0][kc + lk][wc + 16 * j + li];
+                bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < FNB / 32; ++j) {
+                    if constexpr (!decltype(full)::value) {
+                        if (i >= ni || j >= nj) continue;
+                    }
+                    accF[i][j] = mfma(ae[i], bs[j], accF[i][j]);
+                    accG[i][j] = mfma(ao[i], bt[j], accG[i][j]);
+                }
+        }
+    };
     load(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
@@ -310,27 +366,10 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
         }
         __syncthreads();
         if (k0 + FK < KI) load(k0 + FK);
-#pragma unroll
-        for (int kc = 0; kc < FK; kc += 4) {
-            double ae[2], ao[2], bs[FNB / 32], bt[FNB / 32];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ae[i] = As[kc + lk][wr + 16 * i + li];
-                ao[i] = As[kc + lk][FM + wr + 16 * i + li];
-            }
-#pragma unroll
-            for (int j = 0; j < FNB / 32; ++j) {
-                bs[j] = Bs[0][kc + lk][wc + 16 * j + li];
-                bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < FNB / 32; ++j) {
-                    accF[i][j] = mfma(ae[i], bs[j], accF[i][j]);
-                    accG[i][j] = mfma(ao[i], bt[j], accG[i][j]);
-                }
-        }
+        if (ni == 2 && nj == FNB / 32 && k0 + FK <= KI)
+            step(k0, std::true_type{});
+        else if (ni > 0 && nj > 0)
+            step(k0, std::false_type{});
         __syncthreads();
     }
     // w_m and w_{-m}; the 4 parts of a translation sit in lanes li..li+3 (li % 4 == 0)
@@ -454,18 +493,10 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             ++nu;
         }
     };
-    load(0);
-    for (int k0 = 0; k0 < KI; k0 += AK) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;
-            Ps[idx & 15][idx >> 4] = pa[e];
-            Qs[idx & 15][idx >> 4] = pb[e];
-        }
-#pragma unroll
-        for (int e = 0; e < 16; ++e) Ys[e][tid & 3][tid >> 2] = py[e];
-        __syncthreads();
-        if (k0 + AK < KI) load(k0 + AK);
+    // A wave whose 16 rows lie wholly in the padding of the last row tile skips the
+    // products (a uniform branch), which leaves the matrix core to the other resident waves
+    const bool live = M - row0 - 16 * w > 0;
+    auto step = [&]() {
 #pragma unroll
         for (int cc = 0; cc < AK; cc += 2) {
             double aP[2], aQ[2], bP[4], bQ[4];
@@ -490,6 +521,20 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
                     accQ[i][j] = mfma(aQ[i], bQ[j], accQ[i][j]);
                 }
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < KI; k0 += AK) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;
+            Ps[idx & 15][idx >> 4] = pa[e];
+            Qs[idx & 15][idx >> 4] = pb[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) Ys[e][tid & 3][tid >> 2] = py[e];
+        __syncthreads();
+        if (k0 + AK < KI) load(k0 + AK);
+        if (live) step();
         __syncthreads();
     }
     // D row 4 v + lk of tile i: n = 8 i + ((4 v + lk) & 7), part = (4 v + lk) >> 3, i.e.
