@@ -80,31 +80,21 @@ __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__rest
     const double ch = sqrt(fmax(0.0, 0.5 * (1.0 + cb)));
     const double sh = sqrt(fmax(0.0, 0.5 * (1.0 - cb)));
     const double sgm = (m & 1) ? -1.0 : 1.0;
-    double *base = wd + T.off_d;
-    // degree n's E block starts at fold_off(n, K); it advances by 2 (min(n,K)+1)(n+1)
-    // per degree, so the offsets are carried instead of evaluated per degree
-    int64_t fo = -1;
-    int fn = -1;
+    // Degree n's E block starts at fold_off(n, K) and holds (min(n,K)+1) rows of n+1;
+    // the O block follows it.  The entry (k, m) of degree n+1 lies
+    // 2 (min(n,K)+1)(n+1) + k past that of degree n, so a pointer is carried from
+    // degree to degree (32-bit increments: box orders below 32000).
+    const int nfirst = n0 == 0 ? 1 : n0;
+    double *pe = wd + T.off_d + fold_off(nfirst, K) + (int64_t)k * (nfirst + 1) + m;
+    const bool m0 = m == 0, k0 = k == 0;
     auto put = [&](int n, double dp, double dm) {
         const int kk = n < K ? n : K;
-        if (fn < 0) {
-            fo = fold_off(n, K);
-        } else {
-            const int kp = fn < K ? fn : K;
-            fo += 2 * (int64_t)(kp + 1) * (fn + 1);
-        }
-        fn = n;
-        const int64_t o = fo + (int64_t)k * (n + 1) + m;
-        double e, od;
-        if (m == 0) {
-            e = 2.0 * dp;
-            od = 0.0;
-        } else {
-            e = dp + sgm * dm;
-            od = (k == 0) ? 0.0 : dp - sgm * dm;
-        }
-        base[o] = e;
-        base[o + (int64_t)(kk + 1) * (n + 1)] = od;
+        const double e = m0 ? 2.0 * dp : dp + sgm * dm;
+        const double od = (m0 || k0) ? 0.0 : dp - sgm * dm;
+        const int ob = (kk + 1) * (n + 1);
+        pe[0] = e;
+        pe[ob] = od;
+        pe += 2 * ob + k;
     };
     double p1, p2, q1, q2;  // chains d_{k,m} and d_{k,-m}: values at n-1 and n-2
     int nstart;
@@ -121,14 +111,18 @@ __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__rest
         nstart = n0 + 1;
     }
     // The three-term recurrence of wigner_step for both chains: they share
-    // sqrt((n^2 - k^2)(n^2 - m^2)), which also serves as the d^{n-2} coefficient of
-    // the next degree, so each degree costs one square root and one division.
+    // s_n = sqrt((n^2 - k^2)(n^2 - m^2)), which also serves as the d^{n-2} coefficient
+    // of the next degree.  One reciprocal square root per degree gives both the divisor
+    // and s_n: with x = (n^2 - k^2)(n^2 - m^2) > 0 (n > max(k, m)) and
+    // r = 1 / sqrt((n-1)^2 x),  1 / ((n-1) s_n) = r  and  s_n = (n-1) x r.
     const double k2 = (double)k * k, m2 = (double)m * m, km = (double)k * m;
     double sprev = sqrt(fmax(0.0, ((nstart - 1.0) * (nstart - 1.0) - k2) * ((nstart - 1.0) * (nstart - 1.0) - m2)));
     for (int n = nstart; n <= L; ++n) {
         const double nn = (double)n;
-        const double scur = sqrt((nn * nn - k2) * (nn * nn - m2));
-        const double inv = 1.0 / ((nn - 1.0) * scur);
+        const double x = (nn * nn - k2) * (nn * nn - m2);
+        const double nm1x = (nn - 1.0) * x;
+        const double inv = rsqrt((nn - 1.0) * nm1x);
+        const double scur = nm1x * inv;
         const double a = (2.0 * nn - 1.0), b = nn * (nn - 1.0) * cb, c2 = nn * sprev;
         const double pn = (a * (b - km) * p1 - c2 * p2) * inv;
         const double qn = (m == 0) ? pn : (a * (b + km) * q1 - c2 * q2) * inv;
