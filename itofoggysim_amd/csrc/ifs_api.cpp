@@ -513,6 +513,8 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     const int64_t total = (int64_t)B.src.size();
     int64_t cap = std::max<int64_t>(1, (int64_t)(c->ws_budget / (size_t)per_col));
     cap = std::min<int64_t>(cap, total);
+    // the class kernels address one degree's 2K+1 rows of Y and Z with 32-bit offsets
+    cap = std::min<int64_t>(cap, std::max<int64_t>(1, (int64_t)INT32_MAX / (4LL * nrhs * (2LL * T0.K + 2))));
     B.n_y = B.n_z = 0;
     // walk the class columns, cutting chunks of at most cap columns
     const int64_t nbox = 2LL * T0.Lt * (T0.Lt + 2);

@@ -103,44 +103,40 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-    const int64_t nsrc = (int64_t)C.Ls * (C.Ls + 2);
-    // gather: thread -> (translation t, inner q), idx = tid + 256 e, t = idx / 16, q = idx % 16
-    const double2 *xs[2];
-    const double2 *ps[2];
-#pragma unroll
-    for (int e = 0; e < 1; ++e) {
-        int t = (tid + 256 * e) >> 4;
-        int j = (col0 >> 2) + t;
-        bool ok = j < NTOT;
-        int tt = ok ? j % C.ncols : 0, rr = ok ? j / C.ncols : 0;  // translation, right-hand side
-        xs[e] = ok ? src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0) : nullptr;  // xs[m]
-        ps[e] = ok ? phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL : nullptr;
-    }
+    const int nsrc = C.Ls * (C.Ls + 2);
+    // Loads without bounds branches.  A thread's table rows (tid >> 4, + 16) and
+    // translation (tid >> 4 of the tile) are clamped into the operands: rows >= M and
+    // columns >= NC only feed outputs that are never stored.  Inner entries m >= KI
+    // occur in the last step only, which clamps them and zeroes their table entries.
+    const int q = tid & 15;
+    const int r0 = min(row0 + (tid >> 4), M - 1), r1 = min(row0 + (tid >> 4) + 16, M - 1);
+    const double *pa[4] = {E + r0 * KI, E + r1 * KI, O + r0 * KI, O + r1 * KI};
+    const int jt = min((col0 >> 2) + (tid >> 4), NTOT - 1);
+    const int tt = jt % C.ncols, rr = jt / C.ncols;  // translation, right-hand side
+    const double2 *xs = src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0);  // xs[m]
+    const double2 *ps = phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL;        // e^{i m phi}
     double ra[4];
-    double2 rap[2], rbp[2], ram[2], rbm[2], rph[2];
-    auto load = [&](int k0) {
+    double2 rap[1], rbp[1], ram[1], rbm[1], rph[1];
+    auto load = [&](int k0, auto tail) {
+        int gq = k0 + q;
+        if constexpr (decltype(tail)::value) gq = min(gq, KI - 1);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            int idx = tid + 256 * e;  // 64 rows (E 32, O 32) x 16 inner
-            int r = idx >> 4, q = idx & 15;
-            int gr = row0 + (r & 31), gq = k0 + q;
-            const double *src = (r < 32) ? E : O;
-            ra[e] = (gr < M && gq < KI) ? src[(int64_t)gr * KI + gq] : 0.0;
+            double v = pa[e][gq];
+            if constexpr (decltype(tail)::value) v = k0 + q < KI ? v : 0.0;
+            ra[e] = v;
         }
-#pragma unroll
-        for (int e = 0; e < 1; ++e) {
-            int gq = k0 + ((tid + 256 * e) & 15);
-            if (gq < KI && xs[e]) {
-                rap[e] = xs[e][gq];
-                rbp[e] = xs[e][gq + nsrc];
-                ram[e] = xs[e][-gq];
-                rbm[e] = xs[e][-gq + nsrc];
-                rph[e] = ps[e][gq];
-            } else {
-                rap[e] = rbp[e] = ram[e] = rbm[e] = make_double2(0.0, 0.0);
-                rph[e] = make_double2(0.0, 0.0);
-            }
-        }
+        rap[0] = xs[gq];
+        rbp[0] = xs[gq + nsrc];
+        ram[0] = xs[-gq];
+        rbm[0] = xs[-gq + nsrc];
+        rph[0] = ps[gq];
+    };
+    auto load_step = [&](int k0) {
+        if (k0 + FK <= KI)
+            load(k0, std::false_type{});
+        else
+            load(k0, std::true_type{});
     };
     // MFMA tiles wholly in the padding of the last row, column or inner tile are
     // skipped per wave: live 16-row tiles and 16-column tiles of this wave
@@ -169,7 +165,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
                 }
         }
     };
-    load(0);
+    load_step(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -209,7 +205,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             Bs[1][q][4 * t + 3] = u1i - sg * v1i;
         }
         __syncthreads();
-        if (k0 + FK < KI) load(k0 + FK);
+        if (k0 + FK < KI) load_step(k0 + FK);
         if (ni == 2 && nj == 2 && k0 + FK <= KI)
             step(k0, std::true_type{});
         else if (ni > 0 && nj > 0)
@@ -292,24 +288,37 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
     // raw loads only: the s/t combination happens at the LDS store, so that the
     // prefetch of the next step stays in flight while the matrix cores work
     double ra[8], rzp[FNB / 16], rzm[FNB / 16];
-    auto load = [&](int k0) {
+    // Loads without bounds branches.  A thread's table row m (tid & 63 of E or O) and
+    // Z column (tid & 63) are clamped into the operands: rows m >= M and columns
+    // >= NC only feed outputs that are never stored.  Inner entries k >= KI occur in
+    // the last step only, which clamps them and zeroes their table entries; z_{-0}
+    // is zeroed at the LDS store of the first step.
+    const double *pa = ((tid & 127) < FM ? E : O) + min(row0 + (tid & (FM - 1)), M - 1);
+    const double *pz = Z + min(col0 + (tid & (FNB - 1)), NC - 1);
+    auto load = [&](int k0, auto tail) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            int idx = tid + 256 * e;  // 16 inner x 2 FM rows (E FM, O FM), coalesced along m
-            int q = idx >> 7, r = idx & 127;
-            int gm = row0 + (r & (FM - 1)), gk = k0 + q;
-            const double *src = (r < FM) ? E : O;
-            ra[e] = (gm < M && gk < KI) ? src[(int64_t)gk * M + gm] : 0.0;
+            const int gk = k0 + (tid >> 7) + 2 * e;  // 16 inner x 2 FM rows (E FM, O FM), coalesced along m
+            if constexpr (decltype(tail)::value) {
+                const double v = pa[min(gk, KI - 1) * M];
+                ra[e] = gk < KI ? v : 0.0;
+            } else {
+                ra[e] = pa[gk * M];
+            }
         }
 #pragma unroll
         for (int e = 0; e < FNB / 16; ++e) {
-            int idx = tid + 256 * e;  // 16 inner x FNB columns
-            int q = idx / FNB, c = idx % FNB;
-            int gk = k0 + q, gc = col0 + c;
-            bool ok = gk < KI && gc < NC;
-            rzp[e] = ok ? Z[(int64_t)(kk + gk) * NCY + gc] : 0.0;
-            rzm[e] = (ok && gk > 0) ? Z[(int64_t)(kk - gk) * NCY + gc] : 0.0;
+            int gk = k0 + (tid >> 6) + 4 * e;  // 16 inner x FNB columns
+            if constexpr (decltype(tail)::value) gk = min(gk, KI - 1);
+            rzp[e] = pz[(kk + gk) * NCY];
+            rzm[e] = pz[(kk - gk) * NCY];
         }
+    };
+    auto load_step = [&](int k0) {
+        if (k0 + FK <= KI)
+            load(k0, std::false_type{});
+        else
+            load(k0, std::true_type{});
     };
     // MFMA tiles wholly in the padding of the last row, column or inner tile are
     // skipped per wave: live 16-row tiles and 16-column tiles of this wave
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
                 }
         }
     };
-    load(0);
+    load_step(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -358,14 +367,16 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
             int idx = tid + 256 * e;
             int q = idx / FNB, c = idx % FNB;
             int gk = k0 + q;
-            // s_k = z_k + (-1)^k z_{-k}, t_k = z_k - (-1)^k z_{-k}; s_0 = z_0, t_0 = 0 (rzm = 0 at k = 0)
+            // s_k = z_k + (-1)^k z_{-k}, t_k = z_k - (-1)^k z_{-k}; s_0 = z_0, t_0 = 0
+            double zm = rzm[e];
+            if (e == 0 && k0 == 0) zm = q == 0 ? 0.0 : zm;  // the load read z_0 for z_{-0}
             double sg = (gk & 1) ? -1.0 : 1.0;
             double f = (C.mirror && (gk & 1)) ? -1.0 : 1.0;  // (-1)^k on s_k and t_k
-            Bs[0][q][c] = f * (rzp[e] + sg * rzm[e]);
-            Bs[1][q][c] = f * (rzp[e] - sg * rzm[e]);
+            Bs[0][q][c] = f * (rzp[e] + sg * zm);
+            Bs[1][q][c] = f * (rzp[e] - sg * zm);
         }
         __syncthreads();
-        if (k0 + FK < KI) load(k0 + FK);
+        if (k0 + FK < KI) load_step(k0 + FK);
         if (ni == 2 && nj == FNB / 32 && k0 + FK <= KI)
             step(k0, std::true_type{});
         else if (ni > 0 && nj > 0)
