@@ -55,22 +55,24 @@ constexpr int RPAD = 2;  // LDS row padding (doubles)
 //   E(pi - b) = sigma E(b),  O(pi - b) = -sigma O(b),
 // so the column entries take (-1)^m (forward) or (-1)^k (back) at the LDS store and
 // the results (-1)^{n+k} (forward) or (-1)^{n+m} (back), with T and G negated.
-// Back product block: 64 rows of each product x FNB = 64 real columns; wave w owns
-// rows 32 (w / 2) .. + 31 and columns 32 (w % 2) .. + 31 of both products, so F
-// and G of one entry meet in the same lane; registers bounded for three waves per
-// SIMD.  Forward product block: 32 rows of each product x FNF2 = 64 columns; waves
-// 0,1 compute E, waves 2,3 O (32 columns each), and S and T meet through LDS in the
-// epilogue (at 64 rows the forward kernel's gather registers push it to one wave
-// per SIMD, which is slower).  The narrow column tiles buy occupancy: a tile's
-// gather is issued one inner step ahead, and three waves per SIMD hide its
-// latency better than two.
-constexpr int FM = 64;
-constexpr int FMF = 32;
+// Back product block: FMB = 64 rows of each product x FNB = 64 real columns; wave w
+// owns rows 32 (w / 2) .. + 31 and columns 32 (w % 2) .. + 31 of both products, so
+// F and G of one entry meet in the same lane; registers bounded for three waves per
+// SIMD (128-row blocks fit only two and were slower).  Forward product block: FMF =
+// 64 rows of each product x FNF2 = 64 columns; waves 0,1 compute E, waves 2,3 O (32
+// columns each), and S and T meet through LDS in the epilogue.  Each row block
+// gathers its column panel again, so tall forward blocks pay: 64 rows at three waves
+// per SIMD took the forward product from 0.75 to 0.65 s per matvec against 32 rows.
+// The narrow column tiles buy occupancy: a tile's operands are loaded one inner step
+// ahead, and three waves per SIMD hide their latency better than two.
+constexpr int FMB = 64;
+constexpr int FMF = 64;
 constexpr int FNF2 = 64;
 constexpr int FNB = 64;
 constexpr int FK = 16;
 
-__global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restrict__ desc,
+template <int RM, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__restrict__ desc,
                                                        const int64_t *__restrict__ src_off,
                                                        const int32_t *__restrict__ pidx,
                                                        const double2 *__restrict__ phase,
@@ -86,11 +88,17 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     const int NTOT = C.ncols * C.nrhs;  // columns: translations x right-hand sides
     const int NC = 4 * NTOT;
     const int NCY = 4 * C.ncy * C.nrhs;
-    const int row0 = blockIdx.y * FMF;
+    constexpr int NI = RM / 16;  // 16-row MFMA tiles of a wave
+    constexpr int NE = RM / 8;   // table entries a thread loads per step (2 RM rows x 16 inner)
+    const int row0 = blockIdx.y * RM;
     const int col0 = blockIdx.x * FNF2;
     if (row0 >= M) return;
-    __shared__ double As[FK][2 * FMF + RPAD];     // columns 0..31: E rows, 32..63: O rows
-    __shared__ double Bs[2][FK][FNF2 + RPAD];      // s, t
+    // LDS: the table tile As (columns 0..RM-1: E rows, RM..2RM-1: O rows) and the s, t
+    // tiles Bs; after the loop the same memory passes T between the waves (X)
+    constexpr int AS = FK * (2 * RM + RPAD), BS = 2 * FK * (FNF2 + RPAD), XS = RM * (FNF2 + RPAD);
+    __shared__ double smem[AS + BS > XS ? AS + BS : XS];
+    double(*As)[2 * RM + RPAD] = reinterpret_cast<double(*)[2 * RM + RPAD]>(smem);
+    double(*Bs)[FK][FNF2 + RPAD] = reinterpret_cast<double(*)[FK][FNF2 + RPAD]>(smem + AS);
     const double *E = wd + C.off_d + fold_off(nu, K);
     const double *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
@@ -98,31 +106,33 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     const int li = lane & 15, lk = lane >> 4;
     const int g = w >> 1;            // 0: E product, 1: O product
     const int wc = (w & 1) * 32;
-    dv4 acc[2][2];
+    dv4 acc[NI][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
     const int nsrc = C.Ls * (C.Ls + 2);
-    // Loads without bounds branches.  A thread's table rows (tid >> 4, + 16) and
-    // translation (tid >> 4 of the tile) are clamped into the operands: rows >= M and
-    // columns >= NC only feed outputs that are never stored.  Inner entries m >= KI
-    // occur in the last step only, which clamps them and zeroes their table entries.
+    // Loads without bounds branches.  A thread's table rows ((tid >> 4) + 16 e of E,
+    // then of O) and translation (tid >> 4 of the tile) are clamped into the operands:
+    // rows >= M and columns >= NC only feed outputs that are never stored.  Inner
+    // entries m >= KI occur in the last step only, which clamps them and zeroes their
+    // table entries.
     const int q = tid & 15;
-    const int r0 = min(row0 + (tid >> 4), M - 1), r1 = min(row0 + (tid >> 4) + 16, M - 1);
-    const double *pa[4] = {E + r0 * KI, E + r1 * KI, O + r0 * KI, O + r1 * KI};
+    int roff[NE / 2];  // row offsets, shared by E and O
+#pragma unroll
+    for (int e = 0; e < NE / 2; ++e) roff[e] = min(row0 + (tid >> 4) + 16 * e, M - 1) * KI;
     const int jt = min((col0 >> 2) + (tid >> 4), NTOT - 1);
     const int tt = jt % C.ncols, rr = jt / C.ncols;  // translation, right-hand side
     const double2 *xs = src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0);  // xs[m]
     const double2 *ps = phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL;        // e^{i m phi}
-    double ra[4];
+    double ra[NE];
     double2 rap[1], rbp[1], ram[1], rbm[1], rph[1];
     auto load = [&](int k0, auto tail) {
         int gq = k0 + q;
         if constexpr (decltype(tail)::value) gq = min(gq, KI - 1);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            double v = pa[e][gq];
+        for (int e = 0; e < NE; ++e) {
+            double v = (e < NE / 2 ? E : O)[roff[e % (NE / 2)] + gq];
             if constexpr (decltype(tail)::value) v = k0 + q < KI ? v : 0.0;
             ra[e] = v;
         }
@@ -140,7 +150,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     };
     // MFMA tiles wholly in the padding of the last row, column or inner tile are
     // skipped per wave: live 16-row tiles and 16-column tiles of this wave
-    const int ni = M - row0 >= 32 ? 2 : (M - row0 + 15) / 16;
+    const int ni = M - row0 >= RM ? NI : (M - row0 + 15) / 16;
     const int ncw = NC - col0 - wc;
     const int nj = ncw >= 32 ? 2 : (ncw <= 0 ? 0 : (ncw + 15) / 16);
     auto step = [&](int k0, auto full) {
@@ -149,13 +159,13 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
             if constexpr (!decltype(full)::value) {
                 if (k0 + kc >= KI) break;
             }
-            double a[2], b[2];
+            double a[NI], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kc + lk][g * 32 + 16 * i + li];
+            for (int i = 0; i < NI; ++i) a[i] = As[kc + lk][g * RM + 16 * i + li];
 #pragma unroll
             for (int j = 0; j < 2; ++j) b[j] = Bs[g][kc + lk][wc + 16 * j + li];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < NI; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     if constexpr (!decltype(full)::value) {
@@ -168,7 +178,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     load_step(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < NE; ++e) {
             int idx = tid + 256 * e;
             As[idx & 15][idx >> 4] = ra[e];
         }
@@ -206,17 +216,17 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         }
         __syncthreads();
         if (k0 + FK < KI) load_step(k0 + FK);
-        if (ni == 2 && nj == 2 && k0 + FK <= KI)
+        if (ni == NI && nj == 2 && k0 + FK <= KI)
             step(k0, std::true_type{});
         else if (ni > 0 && nj > 0)
             step(k0, std::false_type{});
         __syncthreads();
     }
     // epilogue: the O-product waves hand T to the E-product waves through LDS
-    double *X = &Bs[0][0][0];  // 32 x (FNF2 + RPAD) doubles
+    double *X = smem;  // RM x (FNF2 + RPAD) doubles
     if (g == 1) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -226,7 +236,7 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
     if (g == 1) return;
     double *Y = wy + rows_off(nu, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             int c = wc + 16 * j + li;
@@ -249,7 +259,8 @@ __global__ __launch_bounds__(256) void k_class_rot_fwd(const GroupDesc *__restri
         }
 }
 
-__global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__restrict__ desc,
+template <int FM, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *__restrict__ desc,
                                                         const int32_t *__restrict__ pidx,
                                                         const double2 *__restrict__ phase,
                                                         const double *__restrict__ wd, const double *__restrict__ wz,
@@ -275,11 +286,13 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int wr = (w >> 1) * 32;
+    constexpr int NI = FM / 32;         // 16-row MFMA tiles of a wave (FM / 2 rows)
+    constexpr int NEA = FM / 8;         // table entries a thread loads per step
+    const int wr = (w >> 1) * (FM / 2);
     const int wc = (w & 1) * (FNB / 2);
-    dv4 accF[2][FNB / 32], accG[2][FNB / 32];
+    dv4 accF[NI][FNB / 32], accG[NI][FNB / 32];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < FNB / 32; ++j) {
             accF[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
@@ -287,18 +300,18 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
         }
     // raw loads only: the s/t combination happens at the LDS store, so that the
     // prefetch of the next step stays in flight while the matrix cores work
-    double ra[8], rzp[FNB / 16], rzm[FNB / 16];
+    double ra[NEA], rzp[FNB / 16], rzm[FNB / 16];
     // Loads without bounds branches.  A thread's table row m (tid & 63 of E or O) and
     // Z column (tid & 63) are clamped into the operands: rows m >= M and columns
     // >= NC only feed outputs that are never stored.  Inner entries k >= KI occur in
     // the last step only, which clamps them and zeroes their table entries; z_{-0}
     // is zeroed at the LDS store of the first step.
-    const double *pa = ((tid & 127) < FM ? E : O) + min(row0 + (tid & (FM - 1)), M - 1);
+    const double *pa = ((tid & (2 * FM - 1)) < FM ? E : O) + min(row0 + (tid & (FM - 1)), M - 1);
     const double *pz = Z + min(col0 + (tid & (FNB - 1)), NC - 1);
     auto load = [&](int k0, auto tail) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int gk = k0 + (tid >> 7) + 2 * e;  // 16 inner x 2 FM rows (E FM, O FM), coalesced along m
+        for (int e = 0; e < NEA; ++e) {
+            const int gk = k0 + tid / (2 * FM) + (256 / (2 * FM)) * e;  // 16 inner x 2 FM rows (E, O), coalesced along m
             if constexpr (decltype(tail)::value) {
                 const double v = pa[min(gk, KI - 1) * M];
                 ra[e] = gk < KI ? v : 0.0;
@@ -323,7 +336,7 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
     // MFMA tiles wholly in the padding of the last row, column or inner tile are
     // skipped per wave: live 16-row tiles and 16-column tiles of this wave
     const int nrw = M - row0 - wr;
-    const int ni = nrw >= 32 ? 2 : (nrw <= 0 ? 0 : (nrw + 15) / 16);
+    const int ni = nrw >= FM / 2 ? NI : (nrw <= 0 ? 0 : (nrw + 15) / 16);
     const int ncw = NC - col0 - wc;
     const int nj = ncw >= FNB / 2 ? FNB / 32 : (ncw <= 0 ? 0 : (ncw + 15) / 16);
     auto step = [&](int k0, auto full) {
@@ -332,9 +345,9 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
             if constexpr (!decltype(full)::value) {
                 if (k0 + kc >= KI) break;
             }
-            double ae[2], ao[2], bs[FNB / 32], bt[FNB / 32];
+            double ae[NI], ao[NI], bs[FNB / 32], bt[FNB / 32];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < NI; ++i) {
                 ae[i] = As[kc + lk][wr + 16 * i + li];
                 ao[i] = As[kc + lk][FM + wr + 16 * i + li];
             }
@@ -344,7 +357,7 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
                 bt[j] = Bs[1][kc + lk][wc + 16 * j + li];
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < NI; ++i)
 #pragma unroll
                 for (int j = 0; j < FNB / 32; ++j) {
                     if constexpr (!decltype(full)::value) {
@@ -358,9 +371,9 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
     load_step(0);
     for (int k0 = 0; k0 < KI; k0 += FK) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < NEA; ++e) {
             int idx = tid + 256 * e;
-            As[idx >> 7][idx & 127] = ra[e];
+            As[idx / (2 * FM)][idx & (2 * FM - 1)] = ra[e];
         }
 #pragma unroll
         for (int e = 0; e < FNB / 16; ++e) {
@@ -377,7 +390,7 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
         }
         __syncthreads();
         if (k0 + FK < KI) load_step(k0 + FK);
-        if (ni == 2 && nj == FNB / 32 && k0 + FK <= KI)
+        if (ni == NI && nj == FNB / 32 && k0 + FK <= KI)
             step(k0, std::true_type{});
         else if (ni > 0 && nj > 0)
             step(k0, std::false_type{});
@@ -386,7 +399,7 @@ __global__ __launch_bounds__(256, 3) void k_class_rot_back(const GroupDesc *__re
     // w_m and w_{-m}; the 4 parts of a translation sit in lanes li..li+3 (li % 4 == 0)
     const int64_t ndst = (int64_t)C.Lt * (C.Lt + 2);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < FNB / 32; ++j)
 #pragma unroll
@@ -599,15 +612,15 @@ void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *sr
     if (stage == 0) {
         int mt = ((h.Ls < K ? h.Ls : K) + 1 + FMF - 1) / FMF;
         dim3 grid((NC + FNF2 - 1) / FNF2, mt, h.Ls);
-        hipLaunchKernelGGL(k_class_rot_fwd, grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
+        hipLaunchKernelGGL((k_class_rot_fwd<FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
         dim3 grid((h.ncols * h.nrhs + AN - 1) / AN, mt, 2 * K + 1);
         hipLaunchKernelGGL(k_class_axial, grid, dim3(256), 0, s, d_desc, wax, wy, wz);
     } else {
-        int mt = (h.Lt + 1 + FM - 1) / FM;
+        int mt = (h.Lt + 1 + FMB - 1) / FMB;
         dim3 grid((NC + FNB - 1) / FNB, mt, h.Lt);
-        hipLaunchKernelGGL(k_class_rot_back, grid, dim3(256), 0, s, d_desc, pidx, phase, wd, wz, (double2 *)wy);
+        hipLaunchKernelGGL((k_class_rot_back<FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd, wz, (double2 *)wy);
     }
 }
 
