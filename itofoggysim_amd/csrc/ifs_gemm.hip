@@ -474,9 +474,13 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     __shared__ double2 Ps[AK][AM + 1];
     __shared__ double2 Qs[AK][AM + 1];
     // Y tile by part: the 16 lanes of an MFMA B fragment read 16 consecutive columns of
-    // one part; a part row of AN + 16 doubles puts the two parts one half-wave reads
-    // 32 banks apart (no bank conflicts on the fragment loads)
-    __shared__ double Ys[AK][4][AN + 16];
+    // one part.  Part p of an inner row starts YP(p) = 80 p (+ 8 for p >= 2) doubles in,
+    // i.e. at bank offsets 0, 32, 16, 48 of the 64 four-byte banks: the two parts one
+    // half-wave reads (0,1 or 2,3) lie 32 banks apart, and the tile store (8 columns x
+    // 4 parts per half-wave) covers all 64 banks once; with parts 80 doubles apart,
+    // parts 0 and 2 shared banks and the store was a 2-way conflict.
+    constexpr int YROW = 3 * 80 + 8 + AN;
+    __shared__ double Ys[AK][YROW];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -534,8 +538,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int col = 16 * j + li;
-                bP[j] = Ys[cc + nul][pin][col];
-                bQ[j] = Ys[cc + nul][2 + pin][col];
+                bP[j] = Ys[cc + nul][80 * pin + col];
+                bQ[j] = Ys[cc + nul][168 + 80 * pin + col];
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             Qs[idx & 15][idx >> 4] = pb[e];
         }
 #pragma unroll
-        for (int e = 0; e < 16; ++e) Ys[e][tid & 3][tid >> 2] = py[e];
+        for (int e = 0; e < 16; ++e) Ys[e][80 * (tid & 3) + ((tid & 2) ? 8 : 0) + (tid >> 2)] = py[e];
         __syncthreads();
         if (k0 + AK < KI) load(k0 + AK);
         if (live) step();
