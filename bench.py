@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES(50) solve to 1e-6")
     ap.add_argument("--solve-max-iter", type=int, default=100)
+    ap.add_argument("--x0", choices=("single", "zero"), default="single",
+                    help="GMRES initial guess: the single-sphere solution T_j p_j, i.e. the right-hand side "
+                         "(lib_mie_multi_sphere.f90:134-138), or zero")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
@@ -167,10 +170,13 @@ def main():
     }
     if not args.no_solve:
         rhs = torch.from_numpy(s.get_vector_b()).to("cuda")
-        xs = torch.zeros_like(rhs)
+        # the single-sphere solution of (I + T A) x = T p is x_j = T_j p_j, the right-hand side itself
+        xs = rhs.clone() if args.x0 == "single" else torch.zeros_like(rhs)
+        torch.cuda.synchronize()
         with torch.cuda.stream(st):
             info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=args.solve_max_iter, tol=1e-6,
-                                  verbose=True)
+                                  use_x0=args.x0 == "single", verbose=True)
+        info["initial_guess"] = "single-sphere solution" if args.x0 == "single" else "zero"
         out["solve"] = info
         out["solves_per_s"] = 1.0 / info["seconds"] if info["converged"] else None
         # one solve per run: its latency is both percentiles (the serving loop measures

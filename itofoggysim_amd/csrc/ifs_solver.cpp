@@ -99,7 +99,12 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         launch_axpby(one, b, mone, r, n, s);
         return IFS_OK;
     };
-    if ((rc = residual())) return rc;
+    // with x = 0 the first residual is b itself (no product)
+    if (opts->use_x0) {
+        if ((rc = residual())) return rc;
+    } else {
+        launch_scale_copy(one, b, r, n, s);
+    }
     double beta = norm2(s, r, n, h, part, rc);
     if (rc) return rc;
     std::vector<cplx> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), hv, hv2;
@@ -240,21 +245,29 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
         if (rc) return rc;
     }
     int total_mv = 0;
-    auto residual = [&]() -> int {
-        int q = matvec_impl(c, X, R, 1, nrhs, n);
-        if (q) return q;
-        ++total_mv;
+    // first: with x = 0 the residual is b itself (no product)
+    auto residual = [&](bool first) -> int {
+        int q = IFS_OK;
+        if (!first || opts->use_x0) {
+            q = matvec_impl(c, X, R, 1, nrhs, n);
+            if (q) return q;
+            ++total_mv;
+        }
         for (int r = 0; r < nrhs; ++r) {
             if (conv[r]) continue;
-            ++iters[r];
-            launch_axpby(one, B + r * n, mone, R + r * n, n, s);
+            if (!first || opts->use_x0) {
+                ++iters[r];
+                launch_axpby(one, B + r * n, mone, R + r * n, n, s);
+            } else {
+                launch_scale_copy(one, B + r * n, R + r * n, n, s);
+            }
             beta[r] = norm2(s, R + r * n, n, h, part, q);
             if (q) return q;
             conv[r] = bnorm[r] == 0.0 || beta[r] / bnorm[r] < opts->tol;
         }
         return IFS_OK;
     };
-    if ((rc = residual())) return rc;
+    if ((rc = residual(true))) return rc;
     struct St {
         std::vector<cplx> H, g, cs, sn;
         int jend = 0;
@@ -352,7 +365,7 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
                 return IFS_E_HIP;
             launch_gemv_add(V + r * ldv, n, S.jend, h, X + r * n, n, s);
         }
-        if ((rc = residual())) return rc;
+        if ((rc = residual(false))) return rc;
         for (int r = 0; r < nrhs; ++r)
             if (brk[r] && conv[r]) brk[r] = 0;
     }
