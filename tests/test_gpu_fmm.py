@@ -143,3 +143,31 @@ def test_fmm_vs_direct_high_order(n, cache):
     assert err < 1e-7
     f.close()
     d.close()
+
+
+@pytest.mark.parametrize("n,leaf,d0", [(37, 4, 6.0), (333, 8, 6.0)])
+def test_fmm_vs_direct_odd_tiles_batched(n, leaf, d0):
+    """Shapes that leave partial tiles everywhere in the class products (run widths,
+    row counts and inner lengths that are not multiples of the tile sizes; one level
+    at n = 37, two at n = 333), for one right-hand side and for three in one batched
+    matvec: each must match the direct matvec in the normalised basis."""
+    c = config_cluster(2, n=n)
+    kw = dict(lmax=c["lmax"])
+    f = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], use_fmm=True, leaf_size=leaf, fmm_d0=d0,
+                          **kw)
+    d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
+    S, mask = _snm_flat(f.n_global, c["lmax"])
+    rng = np.random.default_rng(n + leaf)
+    X = np.stack([((rng.standard_normal(f.n) + 1j * rng.standard_normal(f.n)) * mask) / S for _ in range(3)])
+    bfb = f.calculate_vector_b_batch(X) * S
+    for r in range(3):
+        xh = X[r] * S
+        bd = d.calculate_vector_b(X[r]) * S
+        bf = f.calculate_vector_b(X[r]) * S
+        den = np.linalg.norm(bd - xh)
+        e1 = np.linalg.norm(bf - bd) / den
+        eb = np.linalg.norm(bfb[r] - bd) / den
+        print(f"n={n} leaf={leaf} rhs {r}: single {e1:.2e}, batched {eb:.2e}")
+        assert e1 < 1e-7 and eb < 1e-7
+    f.close()
+    d.close()
