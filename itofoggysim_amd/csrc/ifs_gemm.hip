@@ -447,12 +447,17 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
 //   (tile row r: n = r % 8, part = r / 8), inner = 2 nu x 2 parts, 16 translations.
 //   Block: 64 rows n x 64 translations; wave w owns rows 16 w .. 16 w + 15.
 constexpr int AM = 64;
-constexpr int AN = 64;
 constexpr int AK = 16;
 
+// AN: translations per block (64; 32 for the last tile of a segment whose remainder
+// is at most 32, which would otherwise leave half of a 64-wide tile empty); col_base:
+// first translation of the launch.
+template <int AN>
 __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict__ desc,
                                                      const double2 *__restrict__ wax, const double *__restrict__ wy,
-                                                     double *__restrict__ wz) {
+                                                     double *__restrict__ wz, int col_base) {
+    constexpr int NJ = AN / 16;        // 16-translation MFMA tiles of a wave
+    constexpr int NY = AN / 4;         // Y entries a thread loads per step (16 inner x 4 AN reals)
     const GroupDesc C = *desc;
     const int K = C.K;
     const int k = (int)blockIdx.z - K;
@@ -462,7 +467,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     const int M = Lt - q + 1;
     const int KI = Ls - q + 1;
     const int row0 = blockIdx.y * AM;
-    const int col0 = blockIdx.x * AN;
+    const int col0 = col_base + blockIdx.x * AN;
     if (row0 >= M) return;
     const int NC = 4 * C.ncy * C.nrhs;                     // row stride of Y and Z
     const int64_t cb = 4 * (int64_t)C.ycol0 * C.nrhs;      // first real column of the segment
@@ -488,18 +493,20 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     const int nul = lk >> 1, pin = lk & 1;   // inner: (nu within 2, input part)
     const bool use_im = pout != pin;
     const double sgn = (pout == 0 && pin == 1) ? -1.0 : 1.0;
-    dv4 accP[2][4], accQ[2][4];
+    dv4 accP[2][NJ], accQ[2][NJ];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NJ; ++j) {
             accP[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
             accQ[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
         }
     double2 pa[4], pb[4];
-    double py[16];
-    const bool ycol_ok = col0 + (tid >> 2) < NTOT;      // this thread's translation column
-    const double *ybase = wy + cb + 4 * (int64_t)col0 + tid;
+    double py[NY];
+    const int ycol = AN == 64 ? tid : tid % (4 * AN);   // this thread's real column of the tile
+    const int yrow = AN == 64 ? 0 : tid / (4 * AN);     // ... and its first inner row
+    const bool ycol_ok = col0 + (ycol >> 2) < NTOT;      // its translation column
+    const double *ybase = wy + cb + 4 * (int64_t)col0 + ycol;
     auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -510,15 +517,22 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             pa[e] = ok ? Pb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
             pb[e] = ok ? Qb[(int64_t)gr * KI + gc] : make_double2(0.0, 0.0);
         }
-        // 8 inner rows nu = q + k0 + e (one per e) x 256 reals (64 translations x 4);
+        // inner rows nu = q + k0 + yrow + (256 / (4 AN)) e x 4 AN reals (AN translations x 4);
         // the row of (nu, k) advances by 2 nu + 2 (nu < K) or 2 K + 1 per degree
-        int nu = q + k0;
+        int nu = q + k0 + yrow;
         int64_t row = rows_off(nu, K) + k + (nu < K ? nu : K);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
+        for (int e = 0; e < NY; ++e) {
             py[e] = (ycol_ok && nu <= Ls) ? ybase[row * NC] : 0.0;
             row += nu < K ? 2 * nu + 2 : 2 * K + 1;
             ++nu;
+            if constexpr (AN < 64) {  // inner rows 256 / (4 AN) apart
+#pragma unroll
+                for (int st = 1; st < 256 / (4 * AN); ++st) {
+                    row += nu < K ? 2 * nu + 2 : 2 * K + 1;
+                    ++nu;
+                }
+            }
         }
     };
     // A wave whose 16 rows lie wholly in the padding of the last row tile skips the
@@ -527,7 +541,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     auto step = [&]() {
 #pragma unroll
         for (int cc = 0; cc < AK; cc += 2) {
-            double aP[2], aQ[2], bP[4], bQ[4];
+            double aP[2], aQ[2], bP[NJ], bQ[NJ];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 int r = w * 16 + 8 * i + nl;
@@ -536,7 +550,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
                 aQ[i] = sgn * (use_im ? qv.y : qv.x);
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < NJ; ++j) {
                 int col = 16 * j + li;
                 bP[j] = Ys[cc + nul][80 * pin + col];
                 bQ[j] = Ys[cc + nul][168 + 80 * pin + col];
@@ -544,7 +558,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < NJ; ++j) {
                     accP[i][j] = mfma(aP[i], bP[j], accP[i][j]);
                     accQ[i][j] = mfma(aQ[i], bQ[j], accQ[i][j]);
                 }
@@ -559,7 +573,12 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             Qs[idx & 15][idx >> 4] = pb[e];
         }
 #pragma unroll
-        for (int e = 0; e < 16; ++e) Ys[e][80 * (tid & 3) + ((tid & 2) ? 8 : 0) + (tid >> 2)] = py[e];
+        for (int e = 0; e < NY; ++e) {
+            if constexpr (AN == 64)
+                Ys[e][80 * (tid & 3) + ((tid & 2) ? 8 : 0) + (tid >> 2)] = py[e];
+            else
+                Ys[yrow + (256 / (4 * AN)) * e][80 * (ycol & 3) + ((ycol & 2) ? 8 : 0) + (ycol >> 2)] = py[e];
+        }
         __syncthreads();
         if (k0 + AK < KI) load(k0 + AK);
         if (live) step();
@@ -576,7 +595,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             int kq = n < K ? n : K;
             double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC + cb;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < NJ; ++j) {
                 int t = col0 + 16 * j + li;
                 if (t >= NTOT) continue;
                 row[4 * t + 0] = accP[i][j][h];
@@ -619,8 +638,13 @@ void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *sr
         hipLaunchKernelGGL((k_class_rot_fwd<FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
-        dim3 grid((h.ncols * h.nrhs + AN - 1) / AN, mt, 2 * K + 1);
-        hipLaunchKernelGGL(k_class_axial, grid, dim3(256), 0, s, d_desc, wax, wy, wz);
+        const int nt = h.ncols * h.nrhs, rem = nt % 64;
+        const int nfull = rem > 0 && rem <= 32 ? nt / 64 : (nt + 63) / 64;  // 64-wide tiles
+        if (nfull > 0)
+            hipLaunchKernelGGL(k_class_axial<64>, dim3(nfull, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax, wy, wz, 0);
+        if (nfull * 64 < nt)  // a remainder of at most 32 translations
+            hipLaunchKernelGGL(k_class_axial<32>, dim3(1, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax, wy, wz,
+                               nfull * 64);
     } else {
         int mt = (h.Lt + 1 + FMB - 1) / FMB;
         dim3 grid((NC + FNB - 1) / FNB, mt, h.Lt);
