@@ -101,11 +101,14 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
     B.chunks.clear();
     set_prof_names(B);
     const size_t n = B.tasks.size();
+    B.gq = n > 0;
+    for (auto &T : B.tasks) B.gq = B.gq && T.kind == IFS_ZSEL_BESSEL_J;
+    const bool gq = B.gq;
     // per-task sizes
-    auto sizes = [](const Task &T, int64_t &nd, int64_t &nsct, int64_t &nax, int64_t &ny, int64_t &nz) {
+    auto sizes = [gq](const Task &T, int64_t &nd, int64_t &nsct, int64_t &nax, int64_t &ny, int64_t &nz) {
         int L = std::max(T.Ls, T.Lt);
         nd = fold_size(L, T.K);
-        nsct = sect_size(T.K, T.Lt + T.Ls + 2);
+        nsct = gq ? gqb_alpha_size(T.Lt, T.Ls, T.K) : sect_size(T.K, T.Lt + T.Ls + 2);
         nax = ax_size(T.Lt, T.Ls, T.K);
         ny = 2 * rows_size(T.Ls, T.K);
         nz = 2 * rows_size(T.Lt, T.K);
@@ -171,6 +174,15 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         cur.max_Lt = std::max(cur.max_Lt, T.Lt);
         cur.max_rows_y = std::max(cur.max_rows_y, frow_size(T.Ls, T.K));
         cur.max_rows_z = std::max(cur.max_rows_z, rows_size(T.Lt, T.K));
+        if (gq) {
+            GqbChunk &g = cur.gqb;
+            g.Q = std::max(g.Q, gqb_nodes(T.t, T.Lt, T.Ls));
+            g.Nr = std::max(g.Nr, std::max(T.Lt + 1, T.Ls));
+            g.maxK = std::max(g.maxK, T.K);
+            g.max_big = std::max(g.max_big, std::max(T.Lt + 1, T.Ls));
+            g.max_small = std::max(g.max_small, std::min(T.Lt + 1, T.Ls));
+            g.max_ent = std::max(g.max_ent, (int64_t)T.Lt * T.Ls);
+        }
         int nent = T.Lt * (T.Lt + 3) / 2;  // back-rotation entries (n, m >= 0)
         cur.rowback_blocks = std::max(cur.rowback_blocks, std::min(64, (nent + 255) / 256));
         double f1, f2, f3;
@@ -192,6 +204,16 @@ int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache) 
         mz = std::max(mz, ch.n_z);
     }
     int rc;
+    if (gq) {
+        // shared Legendre table per chunk and the chunk's node set
+        size_t vb = 0;
+        for (auto &ch : B.chunks) {
+            const double *mu, *w;
+            if ((rc = gq_rule(c, ch.gqb.Q, &mu, &w)) != IFS_OK) return rc;
+            vb = std::max(vb, (size_t)(ch.gqb.maxK + 1) * (ch.gqb.Nr + 1) * ch.gqb.Q * 8);
+        }
+        if ((rc = c->ws_gq.alloc(std::max(c->ws_gq.bytes, vb))) != IFS_OK) return rc;
+    }
     if (B.cached) {
         if ((rc = B.own_d.alloc(md * 8)) || (rc = B.own_s.alloc(ms * 16)) || (rc = B.own_ax.alloc(ma * 16)) ||
             (rc = B.own_y.alloc(my * 16)) || (rc = B.own_z.alloc(mz * 16)))
@@ -244,17 +266,23 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
         // one right-hand side and tables not kept: the coaxial coefficients are applied as
         // they are generated and never stored (the table round trip through HBM is what
         // the separate generate/apply pair spends its time on)
-        const bool fused = nrhs == 1 && !B.cached && ch.max_Lt < (ch.max_nmax + 2 <= 128 ? 256 : 1024);
+        const bool fused =
+            !B.gq && nrhs == 1 && !B.cached && ch.max_Lt < (ch.max_nmax + 2 <= 128 ? 256 : 1024);
         if (!(B.cached && B.tables_ready)) {
             {
                 Scope s(c, B.pn[0]);
                 launch_gen_wigner_fold(tt, nt, ch.maxpairs, wd, c->stream);
             }
-            {
+            if (B.gq) {
+                Scope s(c, B.pn[2]);
+                const double *mu = nullptr, *w = nullptr;
+                (void)gq_rule(c, ch.gqb.Q, &mu, &w);  // created at finalize
+                launch_gen_axial_gqb(tt, nt, ch.gqb, mu, w, c->ws_gq.as<double>(), wsc, wax, c->stream);
+            } else {
                 Scope s(c, B.pn[1]);
                 launch_gen_sect(tt, nt, ch.max_nmax, wsc, c->stream);
             }
-            if (!fused) {
+            if (!fused && !B.gq) {
                 Scope s(c, B.pn[2]);
                 launch_gen_axial(tt, nt, ch.maxK, ch.max_nmax, wsc, wax, c->stream);
             }

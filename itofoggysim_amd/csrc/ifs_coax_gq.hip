@@ -297,6 +297,160 @@ __global__ void k_gq_vector(int m0, int nm, int Lt, int Ls, int K, double t, con
     }
 }
 
+// Regular (j_n) coaxial tables of the generic translations (sphere <-> box, one
+// table per task) by the same quadrature, batched over the tasks of a chunk.  The
+// tasks share one node set and one table V of Pb_n^m at the nodes (k_gq_leg); per
+// task only the plane-wave factor w_q exp(i t mu_q) differs:
+//   I(n, nu) = sum_q V_n(mu_q) V_nu(mu_q) w_q exp(i t mu_q),   alpha = i^{n-nu} I.
+// I is symmetric in (n, nu), so a tile runs its 256 rows along the larger of the two
+// index ranges (n <= Lt + 1 for sphere -> box, nu <= Ls for box -> sphere) and its 16
+// columns along the smaller one.  Thread: 4 rows x 4 columns; inner step 16 nodes.
+// alpha of task T, order m: walpha[T.off_s + (m (Lt + 2) + n) (Ls + 1) + nu].
+constexpr int GB_R = 256;
+constexpr int GB_C = 16;
+constexpr int GB_K = 16;
+__global__ __launch_bounds__(256) void k_gqb_alpha(const Task *__restrict__ tasks, int Q, int Nr, int rtiles,
+                                                   int ctiles, const double *__restrict__ mu,
+                                                   const double *__restrict__ w, const double *__restrict__ V,
+                                                   double2 *__restrict__ walpha) {
+    const int ntile = rtiles * ctiles;
+    const int ti = blockIdx.x / ntile, tile = blockIdx.x - ti * ntile;
+    const int m = blockIdx.y;
+    const Task T = tasks[ti];
+    if (m > T.K) return;
+    const int Lt = T.Lt, Ls = T.Ls;
+    const bool rows_n = Lt + 1 >= Ls;
+    const int nbig = rows_n ? Lt + 1 : Ls;
+    const int nsml = rows_n ? Ls : Lt + 1;
+    const int r0 = m + (tile / ctiles) * GB_R, c0 = m + (tile % ctiles) * GB_C;
+    if (r0 > nbig || c0 > nsml) return;
+    __shared__ double Vr[GB_K][GB_R];
+    __shared__ double Vc[GB_K][GB_C];
+    __shared__ double2 Ts[GB_K];
+    const double *Vm = V + (int64_t)m * (Nr + 1) * Q;
+    const int tid = threadIdx.x;
+    const int rg = (tid >> 2) * 4, cg = (tid & 3) * 4;
+    double br[4][4], bi[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) br[a][b] = bi[a][b] = 0.0;
+    for (int k0 = 0; k0 < Q; k0 += GB_K) {
+#pragma unroll
+        for (int e = 0; e < GB_R * GB_K / 256; ++e) {
+            const int idx = tid + 256 * e;  // 256 rows x 16 nodes, nodes fastest
+            const int rr = idx >> 4, qq = idx & 15;
+            const int gq = k0 + qq, gr = r0 + rr;
+            Vr[qq][rr] = (gq < Q && gr <= nbig) ? Vm[(int64_t)gr * Q + gq] : 0.0;
+        }
+        {
+            const int cc = tid >> 4, qq = tid & 15;
+            const int gq = k0 + qq, gc = c0 + cc;
+            Vc[qq][cc] = (gq < Q && gc <= nsml) ? Vm[(int64_t)gc * Q + gq] : 0.0;
+        }
+        if (tid < GB_K) {
+            const int gq = k0 + tid;
+            if (gq < Q) {
+                double s, c;
+                sincos(T.t * mu[gq], &s, &c);
+                Ts[tid] = make_double2(w[gq] * c, w[gq] * s);
+            } else {
+                Ts[tid] = make_double2(0.0, 0.0);
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int qq = 0; qq < GB_K; ++qq) {
+            const double2 tq = Ts[qq];
+            double vr[4], vi[4], vc[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double v = Vr[qq][rg + a];
+                vr[a] = v * tq.x;
+                vi[a] = v * tq.y;
+                vc[a] = Vc[qq][cg + a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    br[a][b] = fma(vr[a], vc[b], br[a][b]);
+                    bi[a][b] = fma(vi[a], vc[b], bi[a][b]);
+                }
+        }
+        __syncthreads();
+    }
+    double2 *al = walpha + T.off_s + (int64_t)m * (Lt + 2) * (Ls + 1);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int ri = r0 + rg + a, ci = c0 + cg + b;
+            if (ri > nbig || ci > nsml) continue;
+            const int n = rows_n ? ri : ci, nu = rows_n ? ci : ri;
+            al[(int64_t)n * (Ls + 1) + nu] = ipowg(((n - nu) % 4 + 4) & 3, make_double2(br[a][b], bi[a][b]));
+        }
+}
+
+// Vector coefficients of the batched tables (as k_gq_vector, per task and order m).
+__global__ void k_gqb_vector(const Task *__restrict__ tasks, int eblk, const double2 *__restrict__ walpha,
+                             double2 *__restrict__ wax) {
+    const int ti = blockIdx.x / eblk, eb = blockIdx.x - ti * eblk;
+    const int m = blockIdx.y;
+    const Task T = tasks[ti];
+    if (m > T.K) return;
+    const int Lt = T.Lt, Ls = T.Ls, K = T.K;
+    const int q = m < 1 ? 1 : m;
+    const int nrow = Lt - q + 1, ncol = Ls - q + 1;
+    const int64_t e = (int64_t)eb * blockDim.x + threadIdx.x;
+    if (nrow <= 0 || ncol <= 0 || e >= (int64_t)nrow * ncol) return;
+    const int n = q + (int)(e / ncol), nu = q + (int)(e % ncol);
+    const int NCc = Ls + 1;
+    const double2 *al = walpha + T.off_s + (int64_t)m * (Lt + 2) * NCc;
+    const double2 c0 = al[(int64_t)n * NCc + nu];
+    const double2 cp = al[(int64_t)(n + 1) * NCc + nu];
+    const double2 cm = (n - 1 >= m) ? al[(int64_t)(n - 1) * NCc + nu] : make_double2(0.0, 0.0);
+    const double t = T.t;
+    const double nn = (double)n;
+    const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1.0));
+    const double f0 = sqrt(nn * (nn + 1.0)), fp = t * coef_ag(n, m) * sqrt(nn / (nn + 1.0)),
+                 fm = t * coef_ag(n - 1, m) * sqrt((nn + 1.0) / nn);
+    const double2 s = make_double2(f0 * c0.x + fp * cp.x + fm * cm.x, f0 * c0.y + fp * cp.y + fm * cm.y);
+    const double2 av = ipowg(((nu - n) % 4 + 4) & 3, make_double2(inv_nu * s.x, inv_nu * s.y));
+    const double fb = t * (double)m * inv_nu / sqrt(nn * (nn + 1.0));
+    const double2 bv = ipowg(((nu - n + 1) % 4 + 4) & 3, make_double2(fb * c0.x, fb * c0.y));
+    const double2 pv = make_double2(av.x + bv.x, av.y + bv.y), qv = make_double2(av.x - bv.x, av.y - bv.y);
+    double2 *Ap = wax + T.off_ax + ax_off(m, Lt, Ls, K);
+    double2 *Bp = Ap + ax_block(m, Lt, Ls);
+    const int64_t o = (int64_t)(n - q) * ncol + (nu - q);
+    Ap[o] = pv;
+    Bp[o] = qv;
+    if (m > 0) {
+        double2 *An = wax + T.off_ax + ax_off(-m, Lt, Ls, K);
+        double2 *Bn = An + ax_block(-m, Lt, Ls);
+        An[o] = qv;
+        Bn[o] = pv;
+    }
+}
+
+int gqb_nodes(double t, int Lt, int Ls) {
+    const int Pdeg = (int)ceil(t + 8.0 * cbrt(t) + 30.0);
+    return (Lt + 1 + Ls + Pdeg) / 2 + 4;
+}
+
+void launch_gen_axial_gqb(const Task *tasks, int ntasks, const GqbChunk &g, const double *mu, const double *w,
+                          double *V, double2 *walpha, double2 *wax, hipStream_t s) {
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(k_gq_leg, dim3((g.Q + 255) / 256, g.maxK + 1), dim3(256), 0, s, 0, g.maxK + 1, g.Nr, g.Q, mu,
+                       V);
+    const int rtiles = (g.max_big + 1 + GB_R - 1) / GB_R, ctiles = (g.max_small + 1 + GB_C - 1) / GB_C;
+    hipLaunchKernelGGL(k_gqb_alpha, dim3((unsigned)ntasks * rtiles * ctiles, g.maxK + 1), dim3(256), 0, s, tasks,
+                       g.Q, g.Nr, rtiles, ctiles, mu, w, V, walpha);
+    const int eblk = (int)((g.max_ent + 255) / 256);
+    hipLaunchKernelGGL(k_gqb_vector, dim3((unsigned)ntasks * eblk, g.maxK + 1), dim3(256), 0, s, tasks, eblk, walpha,
+                       wax);
+}
+
 // |h1_p(t)|, p = 0..P, on the host (Miller's recurrence for j_p, upward for y_p).
 static void hankel_abs_host(double x, int P, std::vector<double> &out) {
     std::vector<double> j(P + 1, 0.0), y(P + 1, 0.0);

@@ -46,6 +46,7 @@ struct Chunk {
     int64_t max_rows_y = 0, max_rows_z = 0;
     int rowback_blocks = 1;
     double f_fwd = 0.0, f_ax = 0.0, f_back = 0.0;  // algorithmic flops of the chunk's products
+    GqbChunk gqb;                    // quadrature tables (batches of regular translations)
 };
 
 // A translation batch: tasks from one source pool into one target pool.
@@ -57,6 +58,10 @@ struct Batch {
     DevBuf own_d, own_s, own_ax, own_y, own_z;  // workspace of a cached batch
     bool cached = false;         // tables generated once and kept
     bool tables_ready = false;
+    // regular (j_n) translations between spheres and boxes: coaxial tables by quadrature
+    // (the recurrence in nu loses up to 1e-6 of the largest entry once the box order
+    // and k|d| pass ~100); alpha takes the place of the sectorial columns at off_s
+    bool gq = false;
     std::string name;
     const char *pn[6] = {"gen_wigner", "gen_sect", "gen_axial", "rot_fwd", "axial_apply", "rot_back"};
 };

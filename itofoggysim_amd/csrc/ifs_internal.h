@@ -149,6 +149,22 @@ GqPlan gq_plan(int kind, double t, int Lt, int Ls, int K, size_t ws_cap_bytes);
 // mu, w: the Q Gauss-Legendre nodes and weights on the device
 void launch_gen_axial_gq(const GqPlan &g, const double *mu, const double *w, void *ws, double2 *wax, hipStream_t s);
 
+// Regular coaxial tables of generic tasks (sphere <-> box) by quadrature, batched:
+// alpha of task T at T.off_s ((K+1) (Lt+2) (Ls+1) complex), tables at T.off_ax.
+struct GqbChunk {
+    int Q = 0;         // shared Gauss-Legendre nodes
+    int Nr = 0;        // Legendre rows of the shared table: max over tasks of max(Lt + 1, Ls)
+    int maxK = 0;
+    int max_big = 0;   // max over tasks of max(Lt + 1, Ls)
+    int max_small = 0; // max over tasks of min(Lt + 1, Ls)
+    int64_t max_ent = 0;  // max over tasks of Lt * Ls
+};
+int gqb_nodes(double t, int Lt, int Ls);
+inline int64_t gqb_alpha_size(int Lt, int Ls, int K) { return (int64_t)(K + 1) * (Lt + 2) * (Ls + 1); }
+// V: (maxK + 1) (Nr + 1) Q doubles of workspace
+void launch_gen_axial_gqb(const Task *tasks, int ntasks, const GqbChunk &g, const double *mu, const double *w,
+                          double *V, double2 *walpha, double2 *wax, hipStream_t s);
+
 // Kernel launchers (ifs_kernels.hip).  All enqueue on `s`.
 void launch_gen_wigner_fold(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s);
 void launch_gen_sect(const Task *tasks, int ntasks, int max_nmax, double2 *wsect, hipStream_t s);

@@ -153,6 +153,18 @@ double ifs_or_fact_ratio(int n, int m) { return (double)fact_ratio_ld(n, m); }
 
 /* ------------------------------------------------------------------ */
 /* Wigner 3j: Schulten-Gordon recurrence in j3, two-sided              */
+/* (K. Schulten, R. G. Gordon, J. Math. Phys. 16 (1975) 1961; the      */
+/* region logic of Luscombe & Luban, Phys. Rev. E 57 (1998) 7274).     */
+/* The family f(j) = (j1 j2 j; m1 m2 m3), j = jmin..jmax, obeys        */
+/*   j A(j+1) f(j+1) + B(j) f(j) + (j+1) A(j) f(j-1) = 0.              */
+/* Near each end of the range f decays towards that end (non-classical */
+/* regions); a recurrence is only run in the direction in which f      */
+/* grows, or inside the oscillating classical region in between:       */
+/* backward from jmax until |f| peaks, forward from jmin until |f|     */
+/* peaks and on through the classical region, matched where both are   */
+/* accurate.  (An earlier version ran the forward pass past a          */
+/* monotonically decaying start, e.g. (20 22 j; -20 22 -2), and lost   */
+/* the sign of the whole family.)                                      */
 /* ------------------------------------------------------------------ */
 static int w3j_all_ld(int j1, int j2, int m1, int m2, ld *f, int *j3min_out)
 {
@@ -172,65 +184,87 @@ static int w3j_all_ld(int j1, int j2, int m1, int m2, ld *f, int *j3min_out)
         f[0] = sgn_top / sqrtl(2.0L * jmin + 1.0L);
         return 1;
     }
+    ld *g = (ld *)calloc(nv, sizeof(ld)); /* backward values, valid on [gb, nv-1] */
+    ld *h = (ld *)calloc(nv, sizeof(ld)); /* forward values, valid on [0, hf] */
 
-    ld *g = (ld *)malloc(sizeof(ld) * nv); /* backward pass */
-    ld *h = (ld *)malloc(sizeof(ld) * nv); /* forward pass */
-
-    /* backward from jmax */
+    /* backward from jmax while |g| grows; two more steps into the classical region */
     g[nv - 1] = 1.0L;
-    {
-        int j = jmax; /* g(j-1) = -(j A(j+1) g(j+1) + B(j) g(j)) / ((j+1) A(j)) ; A(jmax+1)=0 */
-        ld gp1 = 0.0L;
-        for (int idx = nv - 1; idx >= 1; --idx, --j) {
-            ld a = W3A(j);
-            ld val = -((ld)j * (j < jmax ? W3A(j + 1) : 0.0L) * gp1 + W3B(j) * g[idx]) / ((ld)(j + 1) * a);
-            gp1 = g[idx];
-            g[idx - 1] = val;
-            if (fabsl(val) > 1e2000L) {
-                for (int k = idx - 1; k < nv; ++k) g[k] *= 1e-2000L;
-                gp1 *= 1e-2000L;
+    int gb = nv - 1, peak_b = nv - 1, extra = -1;
+    for (int i = nv - 1; i >= 1; --i) {
+        int j = jmin + i;
+        ld gp1 = (i + 1 < nv) ? g[i + 1] : 0.0L;
+        ld a = W3A(j);
+        if (a == 0.0L) break;
+        g[i - 1] = -((ld)j * (i + 1 < nv ? W3A(j + 1) : 0.0L) * gp1 + W3B(j) * g[i]) / ((ld)(j + 1) * a);
+        gb = i - 1;
+        if (fabsl(g[i - 1]) > 1e2000L) {
+            for (int k = i - 1; k < nv; ++k) g[k] *= 1e-2000L;
+        }
+        if (extra < 0) {
+            if (fabsl(g[i - 1]) >= fabsl(g[i])) {
+                peak_b = i - 1;
+            } else {
+                extra = 2;
             }
         }
+        if (extra >= 0 && extra-- == 0) break;
     }
 
-    /* forward from jmin, only when jmin > 0 (j A(j+1) vanishes at j = 0) */
-    int jmid_idx = 0;
-    int use_forward = (jmin > 0);
-    if (use_forward) {
-        h[0] = 1.0L;
-        ld hm1 = 0.0L;
-        int idx;
-        for (idx = 0; idx < nv - 1; ++idx) {
-            int j = jmin + idx;
-            ld val = -(W3B(j) * h[idx] + (ld)(j + 1) * W3A(j) * hm1) / ((ld)j * W3A(j + 1));
-            hm1 = h[idx];
-            h[idx + 1] = val;
-            if (fabsl(val) > 1e2000L) {
-                for (int k = 0; k <= idx + 1; ++k) h[k] *= 1e-2000L;
-                hm1 *= 1e-2000L;
-            }
-            /* stop at the first local maximum of |h|: the classical region starts */
-            if (idx >= 1 && fabsl(h[idx + 1]) < fabsl(h[idx]) && fabsl(h[idx]) >= fabsl(h[idx - 1]) && h[idx] != 0.0L)
-                break;
-        }
-        jmid_idx = idx < nv - 1 ? idx : nv - 1;
-    }
-
-    if (!use_forward || jmid_idx == 0) {
-        for (int i = 0; i < nv; ++i) f[i] = g[i];
+    /* forward from jmin while |h| grows, then on to the backward peak */
+    h[0] = 1.0L;
+    int hf = 0, peak_f = 0;
+    if (jmin == 0) {
+        /* j1 = j2, m3 = 0: (j j 1; m -m 0) / (j j 0; m -m 0) = -m1 ... up to a sign
+           fixed by the recurrence at j = 1; take the ratio from the exact forms
+           (j j 0; m -m 0) = (-1)^(j-m)/sqrt(2j+1), (j j 1; m -m 0) = (-1)^(j-m) m/sqrt(j(j+1)(2j+1)) */
+        h[1] = (j1 > 0) ? (ld)m1 / sqrtl((ld)j1 * (j1 + 1)) : 0.0L;
+        hf = 1;
     } else {
-        /* least-squares match on a window around jmid */
-        ld num = 0.0L, den = 0.0L;
-        int lo = imax(0, jmid_idx - 2), hi = imin(nv - 1, jmid_idx + 1);
-        for (int i = lo; i <= hi; ++i) {
-            num += h[i] * g[i];
-            den += g[i] * g[i];
+        h[1] = -W3B(jmin) * h[0] / ((ld)jmin * W3A(jmin + 1));
+        hf = 1;
+    }
+    if (fabsl(h[1]) >= fabsl(h[0])) peak_f = 1;
+    int growing = fabsl(h[1]) >= fabsl(h[0]);
+    int target = peak_b; /* the forward pass must reach the backward peak (or its own) */
+    for (int i = 1; i < nv - 1; ++i) {
+        if (!growing && i >= target) break;
+        int j = jmin + i;
+        ld den = (ld)j * W3A(j + 1);
+        if (den == 0.0L) break;
+        h[i + 1] = -(W3B(j) * h[i] + (ld)(j + 1) * W3A(j) * h[i - 1]) / den;
+        hf = i + 1;
+        if (fabsl(h[i + 1]) > 1e2000L) {
+            for (int k = 0; k <= i + 1; ++k) h[k] *= 1e-2000L;
         }
-        ld lam = den > 0.0L ? num / den : 0.0L;
-        for (int i = 0; i < nv; ++i) f[i] = (i <= jmid_idx) ? h[i] : lam * g[i];
-        if (lam == 0.0L) {
-            for (int i = 0; i < nv; ++i) f[i] = g[i];
+        if (growing) {
+            if (fabsl(h[i + 1]) >= fabsl(h[i])) {
+                peak_f = i + 1;
+            } else {
+                growing = 0;
+                if (peak_f >= peak_b) break; /* the two accurate ranges overlap */
+            }
         }
+    }
+
+    /* match: both passes are accurate on [max(gb, ...), hf] around the backward peak */
+    int im;
+    if (peak_f >= peak_b) {
+        im = peak_b; /* overlap: both accurate from peak_b to peak_f */
+    } else {
+        im = imin(peak_b, hf);
+    }
+    if (im < gb) im = gb;
+    if (im > hf) im = hf;
+    ld num = 0.0L, den = 0.0L;
+    for (int i = imax(gb, im - 2); i <= imin(hf, im + 2); ++i) {
+        num += h[i] * g[i];
+        den += g[i] * g[i];
+    }
+    ld lam = den > 0.0L ? num / den : 0.0L;
+    if (lam == 0.0L || hf < gb) {
+        for (int i = 0; i < nv; ++i) f[i] = (i >= gb) ? g[i] : 0.0L;
+    } else {
+        for (int i = 0; i < nv; ++i) f[i] = (i <= im) ? h[i] : lam * g[i];
     }
     free(g);
     free(h);

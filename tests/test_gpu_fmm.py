@@ -3,6 +3,7 @@ box truncation) and agreement with the direct matvec within the FMM truncation."
 import numpy as np
 import pytest
 
+from helpers import snm_flat
 from itofoggysim_amd import MultiSphereSolver, config_cluster
 
 pytestmark = pytest.mark.gpu
@@ -104,17 +105,6 @@ def test_fmm_table_cache_modes_config1():
         s_.close()
 
 
-def _snm_flat(N, lmax):
-    """Per-entry factors s_nm of the normalised basis over the flat vector, and the
-    mask of the rows n <= L_j."""
-    from scipy.special import gammaln
-    nn = np.array([q for q in range(1, N + 1) for m in range(-q, q + 1)], dtype=float)
-    mm = np.array([m for q in range(1, N + 1) for m in range(-q, q + 1)], dtype=float)
-    s = np.exp(0.5 * (np.log(nn * (nn + 1) * (2 * nn + 1)) + gammaln(nn - mm + 1) - gammaln(nn + mm + 1)))
-    mask = np.concatenate([np.tile(nn <= min(l, N), 2) for l in lmax])
-    return np.tile(np.concatenate([s, s]), len(lmax)), mask
-
-
 @pytest.mark.parametrize("n,cache", [(200, 0), (600, 0), (200, -1)])
 def test_fmm_vs_direct_high_order(n, cache):
     """Clusters of the 2000-sphere generator with n spheres: one MLFMM level with box
@@ -130,7 +120,7 @@ def test_fmm_vs_direct_high_order(n, cache):
     d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
     counts, levels = f.fmm_stats()
     assert max(levels.values()) > 300
-    S, mask = _snm_flat(f.n_global, c["lmax"])
+    S, mask = snm_flat(f.n_global, c["lmax"])
     rng = np.random.default_rng(n)
     xh = (rng.standard_normal(f.n) + 1j * rng.standard_normal(f.n)) * mask
     x = xh / S
@@ -156,7 +146,7 @@ def test_fmm_vs_direct_odd_tiles_batched(n, leaf, d0):
     f = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], use_fmm=True, leaf_size=leaf, fmm_d0=d0,
                           **kw)
     d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], **kw)
-    S, mask = _snm_flat(f.n_global, c["lmax"])
+    S, mask = snm_flat(f.n_global, c["lmax"])
     rng = np.random.default_rng(n + leaf)
     X = np.stack([((rng.standard_normal(f.n) + 1j * rng.standard_normal(f.n)) * mask) / S for _ in range(3)])
     bfb = f.calculate_vector_b_batch(X) * S

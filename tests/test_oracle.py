@@ -103,6 +103,20 @@ def test_wigner3j_exact_racah(oracle):
         assert abs(oracle.wigner3j(j1, j2, j3, m1, m2, m3) - _w3j_exact(j1, j2, j3, m1, m2, m3)) < 1e-15
 
 
+def test_wigner3j_monotone_families_exact(oracle):
+    """Families whose lower non-classical region decays monotonically (stretched or
+    nearly stretched m's, e.g. (20 22 j; -20 22 -2)): a two-sided recurrence that runs
+    forward through such a region returns them with the wrong sign.  Exact Racah values
+    up to j1 + j2 = 100."""
+    fams = [(20, 22, -20, 22), (20, 22, 20, -22), (40, 12, -40, 12), (48, 50, -48, -2), (33, 17, 33, -17),
+            (25, 25, 25, -25), (30, 45, -29, 44), (50, 50, -50, 3), (12, 40, 12, -40), (7, 31, -7, 0)]
+    for j1, j2, m1, m2 in fams:
+        m3 = -m1 - m2
+        for j3 in range(max(abs(j1 - j2), abs(m3)), j1 + j2 + 1):
+            ex = _w3j_exact(j1, j2, j3, m1, m2, m3)
+            assert abs(oracle.wigner3j(j1, j2, j3, m1, m2, m3) - ex) < 1e-15 + 1e-13 * abs(ex), (j1, j2, j3, m1, m2)
+
+
 # ---------------------------------------------------------------- Xu a, b
 def test_xu_ab_golden(oracle):
     g = REFV["xu_ab"]
@@ -226,6 +240,29 @@ def test_translation_regular_composition(oracle):
     lhs = T(d1, N, Nm) @ T(d2, Nm, N)
     rhs = T(d1 + d2, N, N)
     assert np.max(np.abs(lhs - rhs)) / np.max(np.abs(rhs)) < 1e-11
+
+
+def test_translation_plane_wave_high_order(oracle):
+    """Closed form at high order: a plane wave expanded about c1 and translated with the
+    regular coefficients to c2 is the plane wave expanded about c2,
+        (A p1 + B q1, B p1 + A q1) = (p2, q2),
+    exact once the source order exceeds the target order plus k|c2 - c1| by a margin.
+    Source orders up to 75 and t = 20 exercise 3j families far beyond the CSV fixtures;
+    compared in the normalised basis."""
+    from oracle import farfield as FF
+    Lt, Ls, t = 20, 75, 20.0
+    c1 = np.zeros(3)
+    d = t * np.array([math.sqrt(1 - 0.09) * math.cos(0.4), math.sqrt(1 - 0.09) * math.sin(0.4), 0.3])
+    P1 = FF.plane_wave(c1[None, :], Ls, 1.0, 0.7, 0.3)
+    P2 = FF.plane_wave(d[None, :], Lt, 1.0, 0.7, 0.3)
+    nS, nT = Ls * (Ls + 2), Lt * (Lt + 2)
+    A, B = oracle.translation_ab(t, math.acos(0.3), 0.4, Lt, Ls, 1)
+    ra = A @ P1[:nS] + B @ P1[nS:]
+    rb = B @ P1[:nS] + A @ P1[nS:]
+    S = _snm(Lt)
+    num = np.linalg.norm(np.concatenate([(ra - P2[:nT]) * S, (rb - P2[nT:]) * S]))
+    den = np.linalg.norm(np.concatenate([P2[:nT] * S, P2[nT:] * S]))
+    assert num / den < 1e-12
 
 
 # ---------------------------------------------------------------- Mie
