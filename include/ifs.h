@@ -137,6 +137,17 @@ int ifs_far_field(ifs_ctx *ctx, const double *x, int64_t n, int32_t ndir, const 
                   double *out);
 int ifs_far_field_device(ifs_ctx *ctx, const void *x_dev, int64_t n, int32_t ndir, const void *theta_dev,
                          const void *phi_dev, void *out_dev);
+/* Far field as ifs_far_field_device, allocating nothing: the per-direction tables go to
+ * a fixed device array of the library, IFS_FF_SCRATCH_ENTRIES complex (32 MiB) on every GPU
+ * the library runs on, held until the process ends; the sphere centres are kept in the
+ * context.  ndir * N(N+2) must not exceed IFS_FF_SCRATCH_ENTRIES (else IFS_E_ARG and nothing
+ * runs: split the directions).  Enqueued on the context stream, no synchronisation.  All
+ * calls on one GPU share the array, so calls that overlap on one GPU race.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define IFS_FF_SCRATCH_ENTRIES (1 << 21)
+int ifs_far_field_device_noalloc(ifs_ctx *ctx, const void *x_dev, int64_t n, int32_t ndir, const void *theta_dev,
+                                 const void *phi_dev, void *out_dev);
 /* Amplitude matrix S1..S4 (Bohren & Huffman 3.12, incidence along +z) from the far
  * fields Fx, Fy of the x- (beta = 0) and y-polarised (beta = pi/2) waves:
  * S[8i..8i+7] = S1, S2, S3, S4 (re, im). */

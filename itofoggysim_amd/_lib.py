@@ -87,6 +87,7 @@ SIGNATURES = {
     "ifs_cross_sections_device": (_i32, [_vp, _vp, _vp, _i64, _vp]),
     "ifs_far_field": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "ifs_far_field_device": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    "ifs_far_field_device_noalloc": (_i32, [_vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "ifs_amplitude_matrix": (_i32, [_i32, _vp, _vp, _vp, _vp]),
     "ifs_mueller": (_i32, [_i32, _vp, _vp]),
     "ifs_matvec_batch_device": (_i32, [_vp, _i32, _vp, _vp, _i64, _i64]),

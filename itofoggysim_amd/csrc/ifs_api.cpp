@@ -868,13 +868,15 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     if ((rc = c->d_Leff.alloc(4 * ns)) || (rc = c->d_active.alloc(4 * ns)) || (rc = c->d_off.alloc(8 * ns)) ||
         (rc = c->d_snm.alloc(8ull * N * (N + 2))) || (rc = c->d_an.alloc(16ull * ns * N)) ||
         (rc = c->d_bn.alloc(16ull * ns * N)) || (rc = c->d_xh.alloc(16ull * c->ncompact)) ||
-        (rc = c->d_acc.alloc(16ull * c->ncompact)))
+        (rc = c->d_acc.alloc(16ull * c->ncompact)) || (rc = c->d_pos.alloc(24ull * ns)))
         return fail(rc);
+    if ((rc = far_field_scratch(c->device, &c->ff_scratch)) != IFS_OK) return fail(rc);
     std::vector<double> snm = make_snm(N);
     if (hipMemcpy(c->d_Leff.p, c->Leff.data(), 4 * ns, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_active.p, c->active.data(), 4 * ns, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_off.p, c->off.data(), 8 * ns, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_snm.p, snm.data(), 8ull * N * (N + 2), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(c->d_snm.p, snm.data(), 8ull * N * (N + 2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_pos.p, c->pos.data(), 24ull * ns, hipMemcpyHostToDevice) != hipSuccess)
         return fail(IFS_E_HIP);
 
     // Mie T-matrix on the device (FP64)

@@ -134,6 +134,8 @@ struct ifs_ctx {
     int64_t nflat = 0;              // complex entries of the flat vector
 
     ifs::DevBuf d_Leff, d_active, d_off, d_snm, d_an, d_bn, d_x, d_b;
+    ifs::DevBuf d_pos;                   // sphere centres (3 ns doubles)
+    void *ff_scratch = nullptr;          // this device's far-field scratch array (see ifs_optics.hip)
     ifs::DevBuf d_xh, d_acc;             // compact normalised input / accumulator
     ifs::DevBuf ws_d, ws_s, ws_ax, ws_y, ws_z;
     ifs::DevBuf ws_gq;                       // quadrature generator workspace
@@ -156,6 +158,9 @@ struct ifs_ctx {
 
 
 namespace ifs {
+// Device address of the far-field scratch array on the current device (looked up
+// once per device); ifs_create calls it with the context's device current.
+int far_field_scratch(int device, void **out);
 // Translations grouped in classes that share one translation vector (one octree
 // offset), run as tiled products (ifs_gemm.hip).  At finalize the classes are
 // ordered by (length, polar angle) and cut into chunks that fit the workspace;

@@ -22,6 +22,8 @@
 
 #include <cmath>
 #include <complex>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "ifs_engine.h"
@@ -260,6 +262,30 @@ void d_pm1_host(int N, long double beta, std::vector<long double> &dp, std::vect
 
 }  // namespace ifs
 
+// Scratch of ifs_far_field_device_noalloc: the per-direction T/P tables, in a
+// fixed array of this library on every GPU it runs on (held until the process
+// ends).  Host code never uses the array's name as an address: its device address
+// is looked up once per device (far_field_scratch) and passed to the kernels.
+static __device__ __align__(256) double2 scratch[IFS_FF_SCRATCH_ENTRIES];
+
+namespace ifs {
+int far_field_scratch(int device, void **out) {
+    static std::mutex mu;
+    static std::map<int, void *> per_device;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = per_device.find(device);
+    if (it != per_device.end()) {
+        *out = it->second;
+        return IFS_OK;
+    }
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return IFS_E_HIP;
+    per_device[device] = p;
+    *out = p;
+    return IFS_OK;
+}
+}  // namespace ifs
+
 using cplx = std::complex<double>;
 
 extern "C" int ifs_plane_wave(ifs_ctx *c, double alpha, double beta, double *p_out, int64_t n) {
@@ -373,6 +399,25 @@ extern "C" int ifs_far_field_device(ifs_ctx *c, const void *x_dev, int64_t n, in
                        tab.as<double2>(), (double2 *)out_dev);
     if (hipGetLastError() != hipSuccess) return IFS_E_HIP;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return IFS_E_HIP;  // tab, dpos are freed on return
+    return IFS_OK;
+}
+
+extern "C" int ifs_far_field_device_noalloc(ifs_ctx *c, const void *x_dev, int64_t n, int32_t ndir,
+                                            const void *theta_dev, const void *phi_dev, void *out_dev) {
+    if (!c || !x_dev || !theta_dev || !phi_dev || !out_dev || n != ifs_vector_length(c) || ndir < 0)
+        return IFS_E_ARG;
+    if (ndir == 0) return IFS_OK;
+    const int N = c->N;
+    if ((int64_t)ndir * N * (N + 2) > (int64_t)IFS_FF_SCRATCH_ENTRIES) return IFS_E_ARG;
+    if (!c->ff_scratch || !c->d_pos.p) return IFS_E_ARG;
+    double2 *tab = (double2 *)c->ff_scratch;  // every entry the far-field kernel reads is written first
+    hipLaunchKernelGGL(ifs::k_ff_tables, dim3(ndir), dim3(256), 0, c->stream, N, (const double *)theta_dev, tab);
+    const size_t shm = sizeof(double2) * ((size_t)ifs::FFD * (2 * N + 1) + 256);
+    hipLaunchKernelGGL(ifs::k_far_field, dim3((ndir + ifs::FFD - 1) / ifs::FFD), dim3(256), shm, c->stream, N, c->ns,
+                       c->d_Leff.as<int32_t>(), c->d_pos.as<double>(), c->prm.k, c->d_snm.as<double>(),
+                       (const double2 *)x_dev, ndir, (const double *)theta_dev, (const double *)phi_dev, tab,
+                       (double2 *)out_dev);
+    if (hipGetLastError() != hipSuccess) return IFS_E_HIP;
     return IFS_OK;
 }
 
