@@ -25,7 +25,17 @@ FP64_PEAK_TFLOPS = 78.6
 METRIC = "MLFMM matvecs/s & solves/s at rel. residual 1e-6; p50/p99 solve latency"
 
 
-def cpu_baseline(cluster, npairs, threads):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(cluster, npairs, threads, affinity):
     from oracle import oracle as O
     ns = len(cluster["radii"])
     N = int(cluster["lmax"].max())
@@ -40,12 +50,52 @@ def cpu_baseline(cluster, npairs, threads):
     dt = time.perf_counter() - t0
     pairs_per_s = npairs / dt
     return {"value": pairs_per_s / (ns * (ns - 1)), "unit": "matvecs/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "sample": f"{npairs} random ordered sphere pairs of the same 2000-sphere cluster through the oracle "
                       f"(full Xu A/B by the Gaunt/3j formula per pair, dense apply) in {dt:.1f} s on {threads} "
                       f"threads, extrapolated to one direct matvec over all {ns * (ns - 1)} ordered pairs"}
 
 
+def solve_latency(seeds, x0, max_iter, log, n=None):
+    """End-to-end request latency on fresh contexts: create (octree, Mie, plan) + first
+    matvec's table generation + GMRES(50) to 1e-6 from the single-sphere solution."""
+    import torch
+    from itofoggysim_amd import MultiSphereSolver, config_cluster
+    runs = []
+    for seed in seeds:
+        c = config_cluster(2, replica=seed - 2000, n=n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"], use_fmm=True,
+                              leaf_size=32, fmm_d0=6.0)
+        t_build = time.perf_counter() - t0
+        rhs = torch.from_numpy(s.get_vector_b()).to("cuda")
+        xs = rhs.clone() if x0 == "single" else torch.zeros_like(rhs)
+        torch.cuda.synchronize()
+        info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=max_iter, tol=1e-6,
+                              use_x0=x0 == "single", verbose=False)
+        total = time.perf_counter() - t0
+        s.close()
+        del rhs, xs
+        runs.append({"seed": seed, "create_s": round(t_build, 3), "solve_s": round(info["seconds"], 3),
+                     "total_s": round(total, 3), "iterations": info["iterations"], "converged": info["converged"],
+                     "rel_residual": info["rel_residual"], "rel_residual_flat": info["rel_residual_flat"]})
+        print(f"# latency seed {seed}: {total:.1f} s end to end ({info['iterations']} matvecs, "
+              f"residual {info['rel_residual']:.2e})", file=log, flush=True)
+    tot = sorted(r["total_s"] for r in runs)
+    # p50 / p99 by linear interpolation over the samples
+    return {"samples": len(runs), "p50_s": round(float(np.percentile(tot, 50)), 3),
+            "p99_s": round(float(np.percentile(tot, 99)), 3),
+            "solves_per_s": round(len(runs) / sum(tot), 5) if all(r["converged"] for r in runs) else None,
+            "runs": runs}
+
+
 def main():
+    # the CPU baseline's OpenMP threads: all cores, bound close (read at libgomp load; the
+    # binding pins the main thread, so the CPU set is read first)
+    affinity = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -58,6 +108,8 @@ def main():
                     help="GMRES initial guess: the single-sphere solution T_j p_j, i.e. the right-hand side "
                          "(lib_mie_multi_sphere.f90:134-138), or zero")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
+    ap.add_argument("--latency-seeds", type=int, default=3,
+                    help="clusters (seeds 2000, 2001, ...) for the end-to-end solve latency; 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -177,15 +229,23 @@ def main():
             info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=args.solve_max_iter, tol=1e-6,
                                   use_x0=args.x0 == "single", verbose=True)
         info["initial_guess"] = "single-sphere solution" if args.x0 == "single" else "zero"
+        info["note"] = "tables generated in the warm-up matvec (steady-state solve)"
         out["solve"] = info
-        out["solves_per_s"] = 1.0 / info["seconds"] if info["converged"] else None
-        # one solve per run: its latency is both percentiles (the serving loop measures
-        # p50/p99 over many requests)
-        out["solve_latency_s"] = {"p50": info["seconds"], "p99": info["seconds"], "samples": 1}
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(c, args.cpu_pairs, threads)
+        out["solves_per_s_steady"] = 1.0 / info["seconds"] if info["converged"] else None
+        del rhs, xs
     s.close()
+    if not args.no_solve and args.latency_seeds > 0:
+        torch.cuda.synchronize()
+        lat = solve_latency([2000 + i for i in range(args.latency_seeds)], args.x0, args.solve_max_iter, sys.stderr,
+                            n=args.n if args.n != 2000 else None)
+        out["solve_latency_s"] = {k: lat[k] for k in ("p50_s", "p99_s", "samples")}
+        out["solves_per_s"] = lat["solves_per_s"]
+        out["solve_latency_runs"] = lat["runs"]
+    if not args.no_cpu:
+        # every core of the host share this process runs on: OMP_NUM_THREADS where the host
+        # sets it (the CPU share of one GPU on a shared node), else all of nproc
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(c, args.cpu_pairs, threads, affinity)
     print(json.dumps(out))
 
 

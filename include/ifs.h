@@ -158,17 +158,22 @@ int ifs_mueller(int32_t ndir, const double *S, double *M);
 typedef struct {
     int32_t restart;      /* Krylov subspace size (reference default 10; the 2000-sphere benchmark uses 50) */
     int32_t max_iter;     /* maximum matvecs */
-    double tol;           /* relative residual ||b - M x|| / ||b|| */
+    double tol;           /* relative residual in the normalised basis (ifs_solve_info.rel_residual) */
     int32_t use_x0;       /* 1: x holds the initial guess on entry */
     int32_t verbose;      /* 1: progress lines on stderr (every 5 matvecs and every restart) */
     int32_t reserved[2];
 } ifs_solve_opts;
 
 typedef struct {
-    int32_t iterations;   /* matvecs performed */
+    int32_t iterations;        /* matvecs performed */
     int32_t converged;
-    double rel_residual;  /* true relative residual at exit, recomputed in FP64 */
+    double rel_residual;       /* true relative residual at exit, recomputed in FP64, in the normalised
+                                  basis x_hat = s_nm x, s_nm = sqrt(n(n+1)(2n+1)(n-m)!/(n+m)!):
+                                  ||S (b - M x)|| / ||S b||, the norm the tolerance refers to */
     double seconds;
+    double rel_residual_flat;  /* the same residual in the caller's flat basis, ||b - M x|| / ||b||
+                                  (the norm of the reference's GMRES driver); entries there span
+                                  1e+-80 at N = 50, so this norm is dominated by the rows m = n */
 } ifs_solve_info;
 
 /* Solves (I + T A) x = rhs with restarted GMRES on the device (FP64 Arnoldi,
@@ -257,8 +262,11 @@ int ifs_profile_enable(ifs_ctx *ctx, int32_t enable);
  * flops is the algorithmic flop count of the recorded launches (0 where not defined). */
 int ifs_profile_read(ifs_ctx *ctx, const char **names, double *total_ms, int64_t *launches, double *flops,
                      int32_t cap);
-/* Algorithmic flop count of one matvec (rotation form (128/3) L^3 per translation,
- * plus T-apply and identity) and compulsory bytes, from the actual plan. */
+/* Algorithmic flop count of one matvec and compulsory bytes, from the actual plan:
+ * per translation the products this library performs (folded rotations
+ * 8 sum_n [(kk+1)(n+1) + kk n] each way, kk = min(n, K); decoupled coaxial shift
+ * 16 sum_k (Lt-q+1)(Ls-q+1), q = max(1,|k|); about 21 L^3 in all), plus the T-apply
+ * and the identity. */
 int ifs_matvec_cost(ifs_ctx *ctx, double *flops, double *bytes);
 
 #ifdef __cplusplus

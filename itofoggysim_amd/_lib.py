@@ -50,7 +50,7 @@ class RequestInfo(ctypes.Structure):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_residual", ctypes.c_double),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("rel_residual_flat", ctypes.c_double)]
 
 
 _vp = ctypes.c_void_p

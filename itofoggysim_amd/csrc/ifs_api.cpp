@@ -994,6 +994,27 @@ int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nr
     return IFS_OK;
 }
 
+// ||S^-1 r_hat|| / ||S^-1 b_hat|| (the relative residual in the flat basis) from the
+// normalised residual and right-hand side on the device, summed on the host in long
+// double (flat entries reach 1e+-80 at N = 50, their squares overflow doubles at N ~ 80).
+double flat_rel_residual(ifs_ctx *c, const double2 *r_hat, const double2 *b_hat, int64_t n) {
+    std::vector<double2> r((size_t)n), b((size_t)n);
+    if (hipMemcpyAsync(r.data(), r_hat, 16ull * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(b.data(), b_hat, 16ull * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return -1.0;
+    const std::vector<double> snm = make_snm(c->N);
+    const int64_t blk = (int64_t)c->N * (c->N + 2);
+    long double rr = 0.0L, bb = 0.0L;
+    for (int64_t i = 0; i < n; ++i) {
+        const long double is = 1.0L / (long double)snm[(size_t)(i % blk)];
+        const long double a = r[i].x * is, d = r[i].y * is, e = b[i].x * is, f = b[i].y * is;
+        rr += a * a + d * d;
+        bb += e * e + f * f;
+    }
+    return bb > 0.0L ? (double)sqrtl(rr / bb) : 0.0;
+}
+
 int scale_flat(ifs_ctx *c, const double2 *x, double2 *y, int inverse) {
     launch_scale_flat(c->ns, c->N, c->d_snm.as<double>(), x, y, inverse, c->stream);
     return hipPeekAtLastError() == hipSuccess ? IFS_OK : IFS_E_HIP;

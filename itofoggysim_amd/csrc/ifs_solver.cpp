@@ -39,6 +39,7 @@ int ctx_N(ifs_ctx *c);
 const double *ctx_pos(ifs_ctx *c);
 double ctx_k(ifs_ctx *c);
 int scale_flat(ifs_ctx *c, const double2 *x, double2 *y, int inverse);
+double flat_rel_residual(ifs_ctx *c, const double2 *r_hat, const double2 *b_hat, int64_t n);
 }  // namespace ifs
 
 namespace {
@@ -88,7 +89,7 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     if (!opts->use_x0) launch_zero(x, n, s);
     if (bnorm == 0.0) {
         launch_zero(x_out, n, s);
-        if (info) *info = {0, 1, 0.0, 0.0};
+        if (info) *info = {0, 1, 0.0, 0.0, 0.0};
         return IFS_OK;
     }
     // r = b - M x
@@ -201,6 +202,8 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         info->converged = converged;
         info->rel_residual = beta / bnorm;
         info->seconds = secs;
+        // r holds the last true residual b_hat - M_hat x_hat
+        info->rel_residual_flat = flat_rel_residual(c, r, b, n);
     }
     if (converged) return IFS_OK;
     return breakdown ? IFS_E_BREAKDOWN : IFS_E_NOCONV;
@@ -232,7 +235,7 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
     const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
     const double2 *bin = (const double2 *)rhs_dev;
     double2 *xout = (double2 *)x_dev;
-    std::vector<double> bnorm(nrhs), beta(nrhs);
+    std::vector<double> bnorm(nrhs), beta(nrhs), flat(nrhs, 0.0);
     std::vector<int> iters(nrhs, 0), conv(nrhs, 0), brk(nrhs, 0);
     for (int r = 0; r < nrhs; ++r) {
         if ((rc = scale_flat(c, bin + r * ld, B + r * n, 0))) return rc;
@@ -264,6 +267,9 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
             beta[r] = norm2(s, R + r * n, n, h, part, q);
             if (q) return q;
             conv[r] = bnorm[r] == 0.0 || beta[r] / bnorm[r] < opts->tol;
+            // the flat-basis residual of this true residual (R of a converged right-hand
+            // side is overwritten by later batched products)
+            flat[r] = bnorm[r] > 0.0 ? flat_rel_residual(c, R + r * n, B + r * n, n) : 0.0;
         }
         return IFS_OK;
     };
@@ -380,6 +386,7 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
             info[r].converged = conv[r];
             info[r].rel_residual = bnorm[r] > 0.0 ? beta[r] / bnorm[r] : 0.0;
             info[r].seconds = secs;
+            info[r].rel_residual_flat = flat[r];
         }
         all = all && conv[r];
         anybrk = anybrk || brk[r];

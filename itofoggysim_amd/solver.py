@@ -145,7 +145,7 @@ class MultiSphereSolver:
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
             check(rc, "ifs_solve")
         return x, dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
-                       seconds=inf.seconds)
+                       rel_residual_flat=inf.rel_residual_flat, seconds=inf.seconds)
 
     def calculate_vector_b_batch(self, X):
         """b_r = (I + T A) x_r for the rows x_r of X [nrhs, n] in one batched pass."""
@@ -167,7 +167,7 @@ class MultiSphereSolver:
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
             check(rc, "ifs_solve_batch")
         return X, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
-                        seconds=i.seconds) for i in infos]
+                        rel_residual_flat=i.rel_residual_flat, seconds=i.seconds) for i in infos]
 
     def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False, verbose=False):
         op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)), int(bool(verbose)))
@@ -177,7 +177,7 @@ class MultiSphereSolver:
         if rc not in (0, -5, -6):
             check(rc, "ifs_solve_device")
         return dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
-                    seconds=inf.seconds, status=rc)
+                    rel_residual_flat=inf.rel_residual_flat, seconds=inf.seconds, status=rc)
 
     # -- instrumentation ------------------------------------------------------
     def set_stream(self, stream_ptr):
