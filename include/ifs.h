@@ -70,8 +70,16 @@ typedef struct {
     int32_t cache_mb;        /* device memory for translation tables kept across matvecs, in MiB
                                 (0: 65% of free memory at creation; < 0: none, tables are
                                 regenerated in every matvec) */
-    int32_t reserved;
+    int32_t precision;       /* ifs_matvec* arithmetic: IFS_PREC_FP64 (0) or IFS_PREC_MIXED (1) */
 } ifs_params;
+
+/* Precision of a matvec or of a solve's Krylov products:
+ *   IFS_PREC_FP64   FP64 throughout;
+ *   IFS_PREC_MIXED  the box translations (M2M, M2L, L2L: the MLFMM's dense products) run
+ *                   as FP32 matrix-core products with FP32 accumulation on the FP64 tables;
+ *                   sphere-level translations, T-apply, vectors and solver state stay FP64. */
+#define IFS_PREC_FP64 0
+#define IFS_PREC_MIXED 1
 
 /* Builds device tables, the Mie T-matrix and (MLFMM) the octree.
  * Replaces lib_mie_ms_solver_constructor + lib_mie_ms_ml_fmm_constructor
@@ -97,6 +105,10 @@ int ifs_matvec_device(ifs_ctx *ctx, const void *x_dev, void *b_dev, int64_t n);
 int ifs_matvec_batch_device(ifs_ctx *ctx, int32_t nrhs, const void *x_dev, void *b_dev, int64_t n, int64_t ld);
 /* Same on host buffers [nrhs][n]. */
 int ifs_matvec_batch(ifs_ctx *ctx, int32_t nrhs, const double *x, double *b, int64_t n);
+
+/* Arithmetic of the context's ifs_matvec* calls from now on (IFS_PREC_FP64 or
+ * IFS_PREC_MIXED; ifs_params.precision at creation).  The tables are shared. */
+int ifs_set_precision(ifs_ctx *ctx, int32_t precision);
 
 /* Launch stream: NULL restores the context's own stream.  The stream must be a
  * default-priority HIP stream. */
@@ -161,7 +173,11 @@ typedef struct {
     double tol;           /* relative residual in the normalised basis (ifs_solve_info.rel_residual) */
     int32_t use_x0;       /* 1: x holds the initial guess on entry */
     int32_t verbose;      /* 1: progress lines on stderr (every 5 matvecs and every restart) */
-    int32_t reserved[2];
+    int32_t precision;    /* IFS_PREC_FP64: every product FP64.  IFS_PREC_MIXED: the Arnoldi products
+                             in mixed precision (ifs_params.precision), the true residual of every
+                             restart cycle in FP64, so the cycles refine the iterate in FP64 until the
+                             FP64 residual meets tol (iterative refinement) */
+    int32_t reserved;
 } ifs_solve_opts;
 
 typedef struct {

@@ -35,12 +35,13 @@ class Params(ctypes.Structure):
     _fields_ = [("k", ctypes.c_double), ("n_global", ctypes.c_int32), ("z_selector", ctypes.c_int32),
                 ("ref_row_zeroing", ctypes.c_int32), ("use_fmm", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
                 ("fmm_d0", ctypes.c_double), ("fmm_lmax_cap", ctypes.c_int32), ("workspace_mb", ctypes.c_int32),
-                ("d_convention", ctypes.c_int32), ("cache_mb", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("d_convention", ctypes.c_int32), ("cache_mb", ctypes.c_int32), ("precision", ctypes.c_int32)]
 
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("restart", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
-                ("use_x0", ctypes.c_int32), ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("use_x0", ctypes.c_int32), ("verbose", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -65,6 +66,7 @@ SIGNATURES = {
     "ifs_matvec": (_i32, [_vp, _vp, _vp, _i64]),
     "ifs_matvec_device": (_i32, [_vp, _vp, _vp, _i64]),
     "ifs_set_stream": (_i32, [_vp, _vp]),
+    "ifs_set_precision": (_i32, [_vp, _i32]),
     "ifs_get_stream": (_vp, [_vp]),
     "ifs_get_mie": (_i32, [_vp, _vp, _vp]),
     "ifs_plane_wave_rhs": (_i32, [_vp, _vp, _i64]),

@@ -682,7 +682,7 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     return IFS_OK;
 }
 
-int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs) {
+int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs, int prec) {
     if (B.gen.empty() || nrhs <= 0) return IFS_OK;
     if (B.plan_nrhs != nrhs) {
         int rc = class_batch_plan(c, B, nrhs);
@@ -734,22 +734,25 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
         (void)gq_rule(c, G.Q, &mu, &w);  // created at plan time
         launch_gen_axial_gq(G, mu, w, c->ws_gq.p, wax + P.g.off_ax, c->stream);
     };
+    const char *pn_fwd = prec == 1 ? "class_rot_fwd_f32" : "class_rot_fwd";
+    const char *pn_ax = prec == 1 ? "class_axial_f32" : "class_axial";
+    const char *pn_back = prec == 1 ? "class_rot_back_f32" : "class_rot_back";
     for (auto &ch : B.chunks) {
         for (auto &P : ch.runs) {
             need_d(P);
-            Scope s(c, "class_rot_fwd", ffwd * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, c->stream);
+            Scope s(c, pn_fwd, ffwd * P.g.ncols * nrhs);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, prec, c->stream);
         }
         for (auto &P : ch.segs) {
             need_ax(P);
-            Scope s(c, "class_axial", fax * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, c->stream);
+            Scope s(c, pn_ax, fax * P.g.ncols * nrhs);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, prec, c->stream);
         }
         for (size_t r = ch.runs.size(); r-- > 0;) {
             auto &P = ch.runs[r];
             need_d(P);
-            Scope s(c, "class_rot_back", fback * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, c->stream);
+            Scope s(c, pn_back, fback * P.g.ncols * nrhs);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, prec, c->stream);
         }
         {
             Scope s(c, "class_scatter");
@@ -906,6 +909,7 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     if (p->cache_mb > 0) c->cache_budget = (size_t)p->cache_mb << 20;
     if (p->cache_mb < 0) c->cache_budget = 0;
     if (p->d_convention != IFS_D_PHYSICAL && p->d_convention != IFS_D_REFERENCE) return fail(IFS_E_ARG);
+    if (p->precision != IFS_PREC_FP64 && p->precision != IFS_PREC_MIXED) return fail(IFS_E_ARG);
     c->dsign = p->d_convention == IFS_D_REFERENCE ? 1.0 : -1.0;
 
     if (!p->use_fmm) {
@@ -956,12 +960,19 @@ int ifs_set_stream(ifs_ctx *c, void *s) {
 
 void *ifs_get_stream(ifs_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int ifs_set_precision(ifs_ctx *c, int32_t precision) {
+    if (!c || (precision != IFS_PREC_FP64 && precision != IFS_PREC_MIXED)) return IFS_E_ARG;
+    c->prm.precision = precision;
+    return IFS_OK;
+}
+
 }  // extern "C"
 
 namespace ifs {
 // b = M x in the reference basis, or b_hat = S M S^-1 x_hat in the normalised
 // basis (normalised = 1), both in the flat layout.
-int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb) {
+int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb,
+                int prec) {
     if (nrhs < 1) return IFS_E_ARG;
     if (ldb < 0) ldb = ld;
     if (nrhs > 1 && (ld < c->nflat || ldb < c->nflat)) return IFS_E_ARG;
@@ -979,7 +990,7 @@ int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nr
         launch_zero(c->d_acc.as<double2>(), nc * nrhs, c->stream);
     }
     if (c->fmm) {
-        rc = fmm::run(c, c->fmm, c->d_xh.as<double2>(), c->d_acc.as<double2>(), nrhs, nc);
+        rc = fmm::run(c, c->fmm, c->d_xh.as<double2>(), c->d_acc.as<double2>(), nrhs, nc, prec);
     } else {
         rc = batch_run(c, c->direct, c->d_xh.as<double2>(), c->d_acc.as<double2>(), nrhs, nc, nc);
     }
@@ -1025,12 +1036,12 @@ extern "C" {
 
 int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
     if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
-    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n);
+    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n, -1, c->prm.precision);
 }
 
 int ifs_matvec_batch_device(ifs_ctx *c, int32_t nrhs, const void *x_dev, void *b_dev, int64_t n, int64_t ld) {
     if (!c || !x_dev || !b_dev || n != c->nflat || nrhs < 1 || ld < n) return IFS_E_ARG;
-    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, nrhs, ld);
+    return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, nrhs, ld, -1, c->prm.precision);
 }
 
 int ifs_matvec_batch(ifs_ctx *c, int32_t nrhs, const double *x, double *b, int64_t n) {
@@ -1038,7 +1049,7 @@ int ifs_matvec_batch(ifs_ctx *c, int32_t nrhs, const double *x, double *b, int64
     int rc;
     if ((rc = c->d_x.alloc(16ull * n * nrhs)) || (rc = c->d_b.alloc(16ull * n * nrhs))) return rc;
     HIP_OK(hipMemcpyAsync(c->d_x.p, x, 16ull * n * nrhs, hipMemcpyHostToDevice, c->stream));
-    if ((rc = matvec_impl(c, c->d_x.as<double2>(), c->d_b.as<double2>(), 0, nrhs, n))) return rc;
+    if ((rc = matvec_impl(c, c->d_x.as<double2>(), c->d_b.as<double2>(), 0, nrhs, n, -1, c->prm.precision))) return rc;
     HIP_OK(hipMemcpyAsync(b, c->d_b.p, 16ull * n * nrhs, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     return IFS_OK;

@@ -209,14 +209,16 @@ struct ClassBatch {
 };
 // b_r = (I + T A) x_r for r < nrhs: x at stride ld, b at stride ldb (-1: ld); normalised
 // != 0 means x and b are in the normalised basis.
-int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb = -1);
+// prec: 0 FP64, 1 FP32 class products (M2M, M2L, L2L); everything else stays FP64.
+int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb = -1,
+                int prec = 0);
 int class_batch_finalize(ifs_ctx *c, ClassBatch &B);
 // Device Gauss-Legendre rule of Q nodes (cached per context): mu, then w.
 int gq_rule(ifs_ctx *c, int Q, const double **mu, const double **w);
 int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs);
 // After every class batch is finalized: decides which tables persist, then plans each.
 int class_batches_prepare(ifs_ctx *c, const std::vector<ClassBatch *> &bs);
-int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs);
+int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs, int prec = 0);
 void class_batch_cost(const ClassBatch &B, double &flops, double &bytes);
 
 int batch_finalize(ifs_ctx *c, Batch &B, size_t budget_bytes, bool allow_cache);

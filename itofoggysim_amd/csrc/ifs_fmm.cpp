@@ -323,7 +323,7 @@ fail:
 
 void destroy(Fmm *f) { delete f; }
 
-int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t ld) {
+int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t ld, int prec) {
     int rc;
     // multipole / local pools hold nrhs copies at stride nC
     if ((rc = f->C.alloc(16ull * f->nC * nrhs)) || (rc = f->D.alloc(16ull * f->nC * nrhs))) return rc;
@@ -333,10 +333,10 @@ int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t l
     launch_zero(D, f->nC * nrhs, c->stream);
     if ((rc = batch_run(c, f->p2m, xh, C, nrhs, ld, st))) return rc;
     for (int l = f->l_max - 1; l >= f->l_min; --l)
-        if ((rc = class_batch_run(c, f->m2m[l], C, C, nrhs))) return rc;
+        if ((rc = class_batch_run(c, f->m2m[l], C, C, nrhs, prec))) return rc;
     for (int l = f->l_min; l <= f->l_max; ++l) {
-        if ((rc = class_batch_run(c, f->m2l[l], C, D, nrhs))) return rc;
-        if (l > f->l_min && (rc = class_batch_run(c, f->l2l[l], D, D, nrhs))) return rc;
+        if ((rc = class_batch_run(c, f->m2l[l], C, D, nrhs, prec))) return rc;
+        if (l > f->l_min && (rc = class_batch_run(c, f->l2l[l], D, D, nrhs, prec))) return rc;
     }
     if ((rc = batch_run(c, f->l2p, D, acc, nrhs, st, ld))) return rc;
     if ((rc = batch_run(c, f->p2p, xh, acc, nrhs, ld, ld))) return rc;

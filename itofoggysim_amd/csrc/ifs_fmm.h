@@ -18,7 +18,8 @@ int build(ifs_ctx *c, Fmm **out);
 void destroy(Fmm *f);
 // acc += (far field via P2M, M2M, M2L, L2L, L2P) + (near field P2P) of xh
 // (libmlfmm/src/lib/lib_ml_fmm.f90:231-262).
-int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t ld);
+// prec: precision of the class products (see launch_class).
+int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t ld, int prec = 0);
 int tree_export(Fmm *f, int64_t *uindex_out, int32_t *levels_out);
 int stats(Fmm *f, int64_t counts[6], int32_t levels_L[32]);
 

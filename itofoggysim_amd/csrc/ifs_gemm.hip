@@ -30,12 +30,35 @@ namespace ifs {
 namespace {
 
 typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
 
-// FP64 matrix-core tile: D(16x16) += A(16x4) B(4x16) with lane l holding
-// A[l%16][l/16], B[l/16][l%16] and D[4v+l/16][l%16], v = 0..3.
-__device__ __forceinline__ dv4 mfma(double a, double b, dv4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+// Matrix-core tile: D(16x16) += A(16x4) B(4x16) with lane l holding A[l%16][l/16] and
+// B[l/16][l%16] in both the FP64 (v_mfma_f64_16x16x4_f64) and the FP32
+// (v_mfma_f32_16x16x4_f32, FP32 accumulation) form.  Their D layouts differ: value v
+// of lane l is D[drow(v, l/16)][l%16] with drow = 4v + l/16 (FP64) or 4(l/16) + v
+// (FP32).  The products run in the compute type T; tables and pools stay FP64 in
+// memory and are converted where they are staged into LDS, so one set of tables
+// serves both.
+template <typename T> struct Mx;
+template <> struct Mx<double> {
+    typedef dv4 v4;
+    static constexpr bool F32 = false;
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int v, int lk) { return 4 * v + lk; }
+};
+template <> struct Mx<float> {
+    typedef fv4 v4;
+    static constexpr bool F32 = true;
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int v, int lk) { return 4 * lk + v; }
+};
+template <typename T> struct V2;
+template <> struct V2<double> { typedef double2 t; };
+template <> struct V2<float> { typedef float2 t; };
 
 __device__ __forceinline__ int fidx(int n, int m) { return n * n - 1 + m + n; }
 
@@ -71,13 +94,14 @@ constexpr int FNF2 = 64;
 constexpr int FNB = 64;
 constexpr int FK = 16;
 
-template <int RM, int MINB>
+template <typename TC, int RM, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__restrict__ desc,
                                                        const int64_t *__restrict__ src_off,
                                                        const int32_t *__restrict__ pidx,
                                                        const double2 *__restrict__ phase,
                                                        const double2 *__restrict__ src_pool,
-                                                       const double *__restrict__ wd, double *__restrict__ wy) {
+                                                       const double *__restrict__ wd, TC *__restrict__ wy) {
+    typedef typename Mx<TC>::v4 acc4;
     const GroupDesc C = *desc;
     const int nu = blockIdx.z + 1;
     if (nu > C.Ls) return;
@@ -96,9 +120,9 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
     // LDS: the table tile As (columns 0..RM-1: E rows, RM..2RM-1: O rows) and the s, t
     // tiles Bs; after the loop the same memory passes T between the waves (X)
     constexpr int AS = FK * (2 * RM + RPAD), BS = 2 * FK * (FNF2 + RPAD), XS = RM * (FNF2 + RPAD);
-    __shared__ double smem[AS + BS > XS ? AS + BS : XS];
-    double(*As)[2 * RM + RPAD] = reinterpret_cast<double(*)[2 * RM + RPAD]>(smem);
-    double(*Bs)[FK][FNF2 + RPAD] = reinterpret_cast<double(*)[FK][FNF2 + RPAD]>(smem + AS);
+    __shared__ TC smem[AS + BS > XS ? AS + BS : XS];
+    TC(*As)[2 * RM + RPAD] = reinterpret_cast<TC(*)[2 * RM + RPAD]>(smem);
+    TC(*Bs)[FK][FNF2 + RPAD] = reinterpret_cast<TC(*)[FK][FNF2 + RPAD]>(smem + AS);
     const double *E = wd + C.off_d + fold_off(nu, K);
     const double *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
@@ -106,11 +130,11 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
     const int li = lane & 15, lk = lane >> 4;
     const int g = w >> 1;            // 0: E product, 1: O product
     const int wc = (w & 1) * 32;
-    dv4 acc[NI][2];
+    acc4 acc[NI][2];
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc4{0, 0, 0, 0};
     const int nsrc = C.Ls * (C.Ls + 2);
     // Loads without bounds branches.  A thread's table rows ((tid >> 4) + 16 e of E,
     // then of O) and translation (tid >> 4 of the tile) are clamped into the operands:
@@ -159,7 +183,7 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
             if constexpr (!decltype(full)::value) {
                 if (k0 + kc >= KI) break;
             }
-            double a[NI], b[2];
+            TC a[NI], b[2];
 #pragma unroll
             for (int i = 0; i < NI; ++i) a[i] = As[kc + lk][g * RM + 16 * i + li];
 #pragma unroll
@@ -171,7 +195,7 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
                     if constexpr (!decltype(full)::value) {
                         if (i >= ni || j >= nj) continue;
                     }
-                    acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                    acc[i][j] = Mx<TC>::mma(a[i], b[j], acc[i][j]);
                 }
         }
     };
@@ -180,7 +204,7 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             int idx = tid + 256 * e;
-            As[idx & 15][idx >> 4] = ra[e];
+            As[idx & 15][idx >> 4] = (TC)ra[e];
         }
 #pragma unroll
         for (int e = 0; e < 1; ++e) {
@@ -205,14 +229,14 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
                 u0r = -u0r, u0i = -u0i, u1r = -u1r, u1i = -u1i;
                 v0r = -v0r, v0i = -v0i, v1r = -v1r, v1i = -v1i;
             }
-            Bs[0][q][4 * t + 0] = u0r + sg * v0r;
-            Bs[0][q][4 * t + 1] = u0i + sg * v0i;
-            Bs[0][q][4 * t + 2] = u1r + sg * v1r;
-            Bs[0][q][4 * t + 3] = u1i + sg * v1i;
-            Bs[1][q][4 * t + 0] = u0r - sg * v0r;
-            Bs[1][q][4 * t + 1] = u0i - sg * v0i;
-            Bs[1][q][4 * t + 2] = u1r - sg * v1r;
-            Bs[1][q][4 * t + 3] = u1i - sg * v1i;
+            Bs[0][q][4 * t + 0] = (TC)(u0r + sg * v0r);
+            Bs[0][q][4 * t + 1] = (TC)(u0i + sg * v0i);
+            Bs[0][q][4 * t + 2] = (TC)(u1r + sg * v1r);
+            Bs[0][q][4 * t + 3] = (TC)(u1i + sg * v1i);
+            Bs[1][q][4 * t + 0] = (TC)(u0r - sg * v0r);
+            Bs[1][q][4 * t + 1] = (TC)(u0i - sg * v0i);
+            Bs[1][q][4 * t + 2] = (TC)(u1r - sg * v1r);
+            Bs[1][q][4 * t + 3] = (TC)(u1i - sg * v1i);
         }
         __syncthreads();
         if (k0 + FK < KI) load_step(k0 + FK);
@@ -223,18 +247,18 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
         __syncthreads();
     }
     // epilogue: the O-product waves hand T to the E-product waves through LDS
-    double *X = smem;  // RM x (FNF2 + RPAD) doubles
+    TC *X = smem;  // RM x (FNF2 + RPAD)
     if (g == 1) {
 #pragma unroll
         for (int i = 0; i < NI; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) X[(16 * i + 4 * v + lk) * (FNF2 + RPAD) + wc + 16 * j + li] = acc[i][j][v];
+                for (int v = 0; v < 4; ++v) X[(16 * i + Mx<TC>::drow(v, lk)) * (FNF2 + RPAD) + wc + 16 * j + li] = acc[i][j][v];
     }
     __syncthreads();
     if (g == 1) return;
-    double *Y = wy + rows_off(nu, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;
+    TC *Y = wy + rows_off(nu, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
@@ -244,27 +268,28 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
             if (gc >= NC) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int r = 16 * i + 4 * v + lk;
+                int r = 16 * i + Mx<TC>::drow(v, lk);
                 int k = row0 + r;
                 if (k >= M) continue;
-                double S = acc[i][j][v], T = X[r * (FNF2 + RPAD) + c];
+                TC S = acc[i][j][v], T = X[r * (FNF2 + RPAD) + c];
                 if (C.mirror) {
-                    double f = ((nu + k) & 1) ? -1.0 : 1.0;
+                    TC f = ((nu + k) & 1) ? -1 : 1;
                     S *= f;
                     T *= -f;
                 }
-                Y[(int64_t)(kk + k) * NCY + gc] = 0.5 * (S + T);
-                if (k > 0) Y[(int64_t)(kk - k) * NCY + gc] = ((k & 1) ? -0.5 : 0.5) * (S - T);
+                Y[(int64_t)(kk + k) * NCY + gc] = (TC)0.5 * (S + T);
+                if (k > 0) Y[(int64_t)(kk - k) * NCY + gc] = ((k & 1) ? (TC)-0.5 : (TC)0.5) * (S - T);
             }
         }
 }
 
-template <int FM, int MINB>
+template <typename TC, int FM, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *__restrict__ desc,
                                                         const int32_t *__restrict__ pidx,
                                                         const double2 *__restrict__ phase,
-                                                        const double *__restrict__ wd, const double *__restrict__ wz,
+                                                        const double *__restrict__ wd, const TC *__restrict__ wz,
                                                         double2 *__restrict__ W) {
+    typedef typename Mx<TC>::v4 acc4;
     const GroupDesc C = *desc;
     const int n = blockIdx.z + 1;
     if (n > C.Lt) return;
@@ -278,11 +303,11 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
     const int row0 = blockIdx.y * FM;
     const int col0 = blockIdx.x * FNB;
     if (row0 >= M) return;
-    __shared__ double As[FK][2 * FM + RPAD];     // columns 0..FM-1: E^T rows, FM..2FM-1: O^T rows
-    __shared__ double Bs[2][FK][FNB + RPAD];     // s, t
+    __shared__ TC As[FK][2 * FM + RPAD];     // columns 0..FM-1: E^T rows, FM..2FM-1: O^T rows
+    __shared__ TC Bs[2][FK][FNB + RPAD];     // s, t
     const double *E = wd + C.off_d + fold_off(n, K);  // E[k][m], KI x M
     const double *O = E + (int64_t)KI * M;
-    const double *Z = wz + rows_off(n, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;  // row kk + k
+    const TC *Z = wz + rows_off(n, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;  // row kk + k
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -290,24 +315,25 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
     constexpr int NEA = FM / 8;         // table entries a thread loads per step
     const int wr = (w >> 1) * (FM / 2);
     const int wc = (w & 1) * (FNB / 2);
-    dv4 accF[NI][FNB / 32], accG[NI][FNB / 32];
+    acc4 accF[NI][FNB / 32], accG[NI][FNB / 32];
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < FNB / 32; ++j) {
-            accF[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-            accG[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+            accF[i][j] = acc4{0, 0, 0, 0};
+            accG[i][j] = acc4{0, 0, 0, 0};
         }
     // raw loads only: the s/t combination happens at the LDS store, so that the
     // prefetch of the next step stays in flight while the matrix cores work
-    double ra[NEA], rzp[FNB / 16], rzm[FNB / 16];
+    double ra[NEA];
+    TC rzp[FNB / 16], rzm[FNB / 16];
     // Loads without bounds branches.  A thread's table row m (tid & 63 of E or O) and
     // Z column (tid & 63) are clamped into the operands: rows m >= M and columns
     // >= NC only feed outputs that are never stored.  Inner entries k >= KI occur in
     // the last step only, which clamps them and zeroes their table entries; z_{-0}
     // is zeroed at the LDS store of the first step.
     const double *pa = ((tid & (2 * FM - 1)) < FM ? E : O) + min(row0 + (tid & (FM - 1)), M - 1);
-    const double *pz = Z + min(col0 + (tid & (FNB - 1)), NC - 1);
+    const TC *pz = Z + min(col0 + (tid & (FNB - 1)), NC - 1);
     auto load = [&](int k0, auto tail) {
 #pragma unroll
         for (int e = 0; e < NEA; ++e) {
@@ -345,7 +371,7 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
             if constexpr (!decltype(full)::value) {
                 if (k0 + kc >= KI) break;
             }
-            double ae[NI], ao[NI], bs[FNB / 32], bt[FNB / 32];
+            TC ae[NI], ao[NI], bs[FNB / 32], bt[FNB / 32];
 #pragma unroll
             for (int i = 0; i < NI; ++i) {
                 ae[i] = As[kc + lk][wr + 16 * i + li];
@@ -363,8 +389,8 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
                     if constexpr (!decltype(full)::value) {
                         if (i >= ni || j >= nj) continue;
                     }
-                    accF[i][j] = mfma(ae[i], bs[j], accF[i][j]);
-                    accG[i][j] = mfma(ao[i], bt[j], accG[i][j]);
+                    accF[i][j] = Mx<TC>::mma(ae[i], bs[j], accF[i][j]);
+                    accG[i][j] = Mx<TC>::mma(ao[i], bt[j], accG[i][j]);
                 }
         }
     };
@@ -373,7 +399,7 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
 #pragma unroll
         for (int e = 0; e < NEA; ++e) {
             int idx = tid + 256 * e;
-            As[idx / (2 * FM)][idx & (2 * FM - 1)] = ra[e];
+            As[idx / (2 * FM)][idx & (2 * FM - 1)] = (TC)ra[e];
         }
 #pragma unroll
         for (int e = 0; e < FNB / 16; ++e) {
@@ -381,10 +407,10 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
             int q = idx / FNB, c = idx % FNB;
             int gk = k0 + q;
             // s_k = z_k + (-1)^k z_{-k}, t_k = z_k - (-1)^k z_{-k}; s_0 = z_0, t_0 = 0
-            double zm = rzm[e];
-            if (e == 0 && k0 == 0) zm = q == 0 ? 0.0 : zm;  // the load read z_0 for z_{-0}
-            double sg = (gk & 1) ? -1.0 : 1.0;
-            double f = (C.mirror && (gk & 1)) ? -1.0 : 1.0;  // (-1)^k on s_k and t_k
+            TC zm = rzm[e];
+            if (e == 0 && k0 == 0) zm = q == 0 ? (TC)0 : zm;  // the load read z_0 for z_{-0}
+            TC sg = (gk & 1) ? -1 : 1;
+            TC f = (C.mirror && (gk & 1)) ? -1 : 1;  // (-1)^k on s_k and t_k
             Bs[0][q][c] = f * (rzp[e] + sg * zm);
             Bs[1][q][c] = f * (rzp[e] - sg * zm);
         }
@@ -405,9 +431,9 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 int c = wc + 16 * j + li;
-                double F = accF[i][j][v], G = accG[i][j][v];
+                double F = (double)accF[i][j][v], G = (double)accG[i][j][v];
                 if (C.mirror) {
-                    double f = ((n + row0 + wr + 16 * i + 4 * v + lk) & 1) ? -1.0 : 1.0;  // (-1)^{n+m}
+                    double f = ((n + row0 + wr + 16 * i + Mx<TC>::drow(v, lk)) & 1) ? -1.0 : 1.0;  // (-1)^{n+m}
                     F *= f;
                     G *= -f;
                 }
@@ -415,7 +441,7 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
                 double p1 = __shfl_down(wp, 1, 64), p2 = __shfl_down(wp, 2, 64), p3 = __shfl_down(wp, 3, 64);
                 double q1 = __shfl_down(wm, 1, 64), q2 = __shfl_down(wm, 2, 64), q3 = __shfl_down(wm, 3, 64);
                 int gc = col0 + c;
-                int m = row0 + wr + 16 * i + 4 * v + lk;
+                int m = row0 + wr + 16 * i + Mx<TC>::drow(v, lk);
                 if ((li & 3) != 0 || gc >= NC || m >= M) continue;
                 int t = gc >> 2;
                 int tt = t % C.ncols, rr = t / C.ncols;  // translation, right-hand side
@@ -452,10 +478,12 @@ constexpr int AK = 16;
 // AN: translations per block (64; 32 for the last tile of a segment whose remainder
 // is at most 32, which would otherwise leave half of a 64-wide tile empty); col_base:
 // first translation of the launch.
-template <int AN>
+template <typename TC, int AN>
 __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict__ desc,
-                                                     const double2 *__restrict__ wax, const double *__restrict__ wy,
-                                                     double *__restrict__ wz, int col_base) {
+                                                     const double2 *__restrict__ wax, const TC *__restrict__ wy,
+                                                     TC *__restrict__ wz, int col_base) {
+    typedef typename Mx<TC>::v4 acc4;
+    typedef typename V2<TC>::t c2;
     constexpr int NJ = AN / 16;        // 16-translation MFMA tiles of a wave
     constexpr int NY = AN / 4;         // Y entries a thread loads per step (16 inner x 4 AN reals)
     const GroupDesc C = *desc;
@@ -476,8 +504,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     const double2 *Qb = Pb + ax_block(k, Lt, Ls);
     // rows padded by one entry: the tile store writes 8 inner columns of one row from
     // 8 consecutive lanes, which a 1 KiB row stride would put on the same banks
-    __shared__ double2 Ps[AK][AM + 1];
-    __shared__ double2 Qs[AK][AM + 1];
+    __shared__ c2 Ps[AK][AM + 1];
+    __shared__ c2 Qs[AK][AM + 1];
     // Y tile by part: the 16 lanes of an MFMA B fragment read 16 consecutive columns of
     // one part.  Part p of an inner row starts YP(p) = 80 p (+ 8 for p >= 2) doubles in,
     // i.e. at bank offsets 0, 32, 16, 48 of the 64 four-byte banks: the two parts one
@@ -485,28 +513,32 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     // 4 parts per half-wave) covers all 64 banks once; with parts 80 doubles apart,
     // parts 0 and 2 shared banks and the store was a 2-way conflict.
     constexpr int YROW = 3 * 80 + 8 + AN;
-    __shared__ double Ys[AK][YROW];
+    __shared__ TC Ys[AK][YROW];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int nl = li & 7, pout = li >> 3;   // A-fragment row: (n within 8, output part)
+    // A-fragment row li -> (n within 8, output part), chosen so that values h and h + 2 of
+    // a lane are the (re, im) parts of one n in both D layouts: FP64 row 4v + lk -> n = 4h + lk;
+    // FP32 row 4 lk + v -> n = h + 2 lk
+    const int nl = Mx<TC>::F32 ? ((li & 1) | ((li >> 2) << 1)) : (li & 7);
+    const int pout = Mx<TC>::F32 ? ((li >> 1) & 1) : (li >> 3);
     const int nul = lk >> 1, pin = lk & 1;   // inner: (nu within 2, input part)
     const bool use_im = pout != pin;
-    const double sgn = (pout == 0 && pin == 1) ? -1.0 : 1.0;
-    dv4 accP[2][NJ], accQ[2][NJ];
+    const TC sgn = (pout == 0 && pin == 1) ? -1 : 1;
+    acc4 accP[2][NJ], accQ[2][NJ];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            accP[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-            accQ[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+            accP[i][j] = acc4{0, 0, 0, 0};
+            accQ[i][j] = acc4{0, 0, 0, 0};
         }
     double2 pa[4], pb[4];
-    double py[NY];
+    TC py[NY];
     const int ycol = AN == 64 ? tid : tid % (4 * AN);   // this thread's real column of the tile
     const int yrow = AN == 64 ? 0 : tid / (4 * AN);     // ... and its first inner row
     const bool ycol_ok = col0 + (ycol >> 2) < NTOT;      // its translation column
-    const double *ybase = wy + cb + 4 * (int64_t)col0 + ycol;
+    const TC *ybase = wy + cb + 4 * (int64_t)col0 + ycol;
     auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -523,7 +555,7 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
         int64_t row = rows_off(nu, K) + k + (nu < K ? nu : K);
 #pragma unroll
         for (int e = 0; e < NY; ++e) {
-            py[e] = (ycol_ok && nu <= Ls) ? ybase[row * NC] : 0.0;
+            py[e] = (ycol_ok && nu <= Ls) ? ybase[row * NC] : (TC)0;
             row += nu < K ? 2 * nu + 2 : 2 * K + 1;
             ++nu;
             if constexpr (AN < 64) {  // inner rows 256 / (4 AN) apart
@@ -541,11 +573,11 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
     auto step = [&]() {
 #pragma unroll
         for (int cc = 0; cc < AK; cc += 2) {
-            double aP[2], aQ[2], bP[NJ], bQ[NJ];
+            TC aP[2], aQ[2], bP[NJ], bQ[NJ];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 int r = w * 16 + 8 * i + nl;
-                double2 p = Ps[cc + nul][r], qv = Qs[cc + nul][r];
+                c2 p = Ps[cc + nul][r], qv = Qs[cc + nul][r];
                 aP[i] = sgn * (use_im ? p.y : p.x);
                 aQ[i] = sgn * (use_im ? qv.y : qv.x);
             }
@@ -559,8 +591,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
-                    accP[i][j] = mfma(aP[i], bP[j], accP[i][j]);
-                    accQ[i][j] = mfma(aQ[i], bQ[j], accQ[i][j]);
+                    accP[i][j] = Mx<TC>::mma(aP[i], bP[j], accP[i][j]);
+                    accQ[i][j] = Mx<TC>::mma(aQ[i], bQ[j], accQ[i][j]);
                 }
         }
     };
@@ -569,8 +601,8 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int idx = tid + 256 * e;
-            Ps[idx & 15][idx >> 4] = pa[e];
-            Qs[idx & 15][idx >> 4] = pb[e];
+            Ps[idx & 15][idx >> 4] = c2{(TC)pa[e].x, (TC)pa[e].y};
+            Qs[idx & 15][idx >> 4] = c2{(TC)pb[e].x, (TC)pb[e].y};
         }
 #pragma unroll
         for (int e = 0; e < NY; ++e) {
@@ -584,16 +616,15 @@ __global__ __launch_bounds__(256) void k_class_axial(const GroupDesc *__restrict
         if (live) step();
         __syncthreads();
     }
-    // D row 4 v + lk of tile i: n = 8 i + ((4 v + lk) & 7), part = (4 v + lk) >> 3, i.e.
-    // v = 0, 2 -> (re, im) of n = 8 i + lk;  v = 1, 3 -> (re, im) of n = 8 i + 4 + lk
+    // values h and h + 2 of a lane (h = 0, 1) are the (re, im) parts of one n of tile i
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            int n = q + row0 + w * 16 + 8 * i + 4 * h + lk;
+            int n = q + row0 + w * 16 + 8 * i + (Mx<TC>::F32 ? h + 2 * lk : 4 * h + lk);
             if (n > Lt) continue;
             int kq = n < K ? n : K;
-            double *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC + cb;
+            TC *row = wz + (rows_off(n, K) + (k + kq)) * (int64_t)NC + cb;
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 int t = col0 + 16 * j + li;
@@ -626,30 +657,44 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterSeg *__restrict__ 
     }
 }
 
-void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
-                  const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
-                  double *wz, int stage, hipStream_t s) {
+template <typename TC>
+static void launch_class_t(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
+                           const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax,
+                           TC *wy, TC *wz, int stage, hipStream_t s) {
     const int NC = 4 * h.ncols * h.nrhs;
     const int K = h.K;
-    if (h.ncols <= 0 || h.nrhs <= 0) return;
     if (stage == 0) {
         int mt = ((h.Ls < K ? h.Ls : K) + 1 + FMF - 1) / FMF;
         dim3 grid((NC + FNF2 - 1) / FNF2, mt, h.Ls);
-        hipLaunchKernelGGL((k_class_rot_fwd<FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool, wd, wy);
+        hipLaunchKernelGGL((k_class_rot_fwd<TC, FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool,
+                           wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
         const int nt = h.ncols * h.nrhs, rem = nt % 64;
         const int nfull = rem > 0 && rem <= 32 ? nt / 64 : (nt + 63) / 64;  // 64-wide tiles
         if (nfull > 0)
-            hipLaunchKernelGGL(k_class_axial<64>, dim3(nfull, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax, wy, wz, 0);
+            hipLaunchKernelGGL((k_class_axial<TC, 64>), dim3(nfull, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax,
+                               (const TC *)wy, wz, 0);
         if (nfull * 64 < nt)  // a remainder of at most 32 translations
-            hipLaunchKernelGGL(k_class_axial<32>, dim3(1, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax, wy, wz,
-                               nfull * 64);
+            hipLaunchKernelGGL((k_class_axial<TC, 32>), dim3(1, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax,
+                               (const TC *)wy, wz, nfull * 64);
     } else {
         int mt = (h.Lt + 1 + FMB - 1) / FMB;
         dim3 grid((NC + FNB - 1) / FNB, mt, h.Lt);
-        hipLaunchKernelGGL((k_class_rot_back<FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd, wz, (double2 *)wy);
+        // W (FP64 columns) aliases the Y workspace, free once the coaxial shifts are done
+        hipLaunchKernelGGL((k_class_rot_back<TC, FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd,
+                           (const TC *)wz, (double2 *)wy);
     }
+}
+
+void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
+                  const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
+                  double *wz, int stage, int prec, hipStream_t s) {
+    if (h.ncols <= 0 || h.nrhs <= 0) return;
+    if (prec == 1)
+        launch_class_t<float>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, (float *)wy, (float *)wz, stage, s);
+    else
+        launch_class_t<double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, wy, wz, stage, s);
 }
 
 void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,

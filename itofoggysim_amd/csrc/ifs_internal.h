@@ -185,12 +185,15 @@ void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, c
 void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblocks, const double *wd,
                      const double2 *wz, double2 *dst_pool, hipStream_t s);
 
+// Precision of the class products (prec): 0 FP64 throughout; 1 FP32 products with FP32
+// accumulation (tables and pools stay FP64 and are converted where staged into LDS;
+// Y and Z hold FP32, the back-rotated columns W FP64).
 // stage 0: forward rotation of a run (source gather with exp(i m phi)) -> Y
 // stage 1: coaxial shift of a segment Y -> Z
 // stage 2: back rotation of a run (exp(-i m phi)) Z -> W (aliases Y)
 void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
                   const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
-                  double *wz, int stage, hipStream_t s);
+                  double *wz, int stage, int prec, hipStream_t s);
 // dst_pool[r dst_stride + seg.dst + e] += sum_{i in [beg,end)} W[r ncy + cols[i]][e],
 // e < nbox = 2 Lt(Lt+2), r < nrhs
 void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,

@@ -107,7 +107,7 @@ void run_request(Req &r) {
     }
     std::vector<ifs_solve_info> info(nd);
     if (!rc) {
-        ifs_solve_opts op{r.restart, r.max_iter, r.tol, 0, 0, {0, 0}};
+        ifs_solve_opts op{r.restart, r.max_iter, r.tol, 0, 0, r.prm.precision, 0};
         rc = ifs_solve_batch(c, nd, rhs.data(), x.data(), n, &op, info.data());
     }
     r.csec.assign(3 * nd, 0.0);

@@ -69,6 +69,9 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c)) return IFS_E_ARG;
     const int m = opts->restart;
     if (m < 1 || opts->max_iter < 1 || !(opts->tol > 0.0)) return IFS_E_ARG;
+    if (opts->precision != IFS_PREC_FP64 && opts->precision != IFS_PREC_MIXED) return IFS_E_ARG;
+    // Arnoldi products in the requested precision; the true residual of every cycle in FP64
+    const int kprec = opts->precision == IFS_PREC_MIXED ? 1 : 0;
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = ctx_stream(c);
     double2 *V, *w, *r, *part, *h;
@@ -118,7 +121,7 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         int jend = 0;
         for (int j = 0; j < m; ++j) {
             double2 *vj = V + (int64_t)j * n;
-            if ((rc = matvec_impl(c, vj, w, 1, 1, n))) return rc;
+            if ((rc = matvec_impl(c, vj, w, 1, 1, n, -1, kprec))) return rc;
             ++iters;
             // CGS with one re-orthogonalisation pass
             if ((rc = dots(s, V, n, j + 1, w, n, h, part, hv))) return rc;
@@ -220,6 +223,8 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
     if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c) || nrhs < 1 || ld < n) return IFS_E_ARG;
     const int m = opts->restart;
     if (m < 1 || opts->max_iter < 1 || !(opts->tol > 0.0)) return IFS_E_ARG;
+    if (opts->precision != IFS_PREC_FP64 && opts->precision != IFS_PREC_MIXED) return IFS_E_ARG;
+    const int kprec = opts->precision == IFS_PREC_MIXED ? 1 : 0;
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = ctx_stream(c);
     DevBuf dV, dW, dR, dB, dX, dpart, dh;
@@ -303,7 +308,7 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
             for (int r = 0; r < nrhs; ++r) work = work || !st[r].done;
             if (!work) break;
             // w_r = M v_{r,j} for every right-hand side in one call
-            if ((rc = matvec_impl(c, V + (int64_t)j * n, W, 1, nrhs, ldv, n))) return rc;
+            if ((rc = matvec_impl(c, V + (int64_t)j * n, W, 1, nrhs, ldv, n, kprec))) return rc;
             ++total_mv;
             for (int r = 0; r < nrhs; ++r) {
                 St &S = st[r];

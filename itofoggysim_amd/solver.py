@@ -14,10 +14,17 @@ from . import _lib
 from ._lib import check, lib
 
 
+def _prec(p):
+    """'fp64' (IFS_PREC_FP64) or 'mixed' (IFS_PREC_MIXED: FP32 matrix-core box translations)."""
+    if p not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    return 0 if p == "fp64" else 1
+
+
 class MultiSphereSolver:
     def __init__(self, centres, radii, m_rel, k, n_global=None, lmax=None, z_selector=3, ref_row_zeroing=False,
                  use_fmm=False, leaf_size=32, fmm_d0=6.0, fmm_lmax_cap=0, workspace_mb=0,
-                 d_convention="physical", cache_mb=0):
+                 d_convention="physical", cache_mb=0, precision="fp64"):
         centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
         radii = np.ascontiguousarray(radii, dtype=np.float64)
         ns = radii.shape[0]
@@ -44,6 +51,7 @@ class MultiSphereSolver:
             raise ValueError("d_convention must be 'physical' or 'reference'")
         pr.d_convention = 0 if d_convention == "physical" else 1
         pr.cache_mb = int(cache_mb)
+        pr.precision = _prec(precision)
         h = ctypes.c_void_p()
         check(lib().ifs_create(ctypes.byref(h), ctypes.byref(cl), ctypes.byref(pr)), "ifs_create")
         self._h = h
@@ -135,11 +143,13 @@ class MultiSphereSolver:
         check(lib().ifs_get_mie(self._h, an.ctypes.data, bn.ctypes.data), "ifs_get_mie")
         return an, bn
 
-    def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True):
-        """Restarted GMRES on the device; returns (x, info dict) (lib_mie_ms_solver_run)."""
+    def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True, precision="fp64"):
+        """Restarted GMRES on the device; returns (x, info dict) (lib_mie_ms_solver_run).
+        precision 'mixed': FP32 box translations in the Arnoldi products, FP64 true residual
+        per cycle (iterative refinement)."""
         b = self.get_vector_b() if rhs is None else np.ascontiguousarray(rhs, dtype=np.complex128)
         x = np.zeros(self.n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
-        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None))
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None), 0, _prec(precision))
         inf = _lib.SolveInfo()
         rc = lib().ifs_solve(self._h, b.ctypes.data, x.ctypes.data, self.n, ctypes.byref(op), ctypes.byref(inf))
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
@@ -156,12 +166,12 @@ class MultiSphereSolver:
         check(lib().ifs_matvec_batch(self._h, X.shape[0], X.ctypes.data, B.ctypes.data, self.n), "ifs_matvec_batch")
         return B
 
-    def solve_batch(self, RHS, restart=50, max_iter=1000, tol=1e-6, raise_on_noconv=True):
+    def solve_batch(self, RHS, restart=50, max_iter=1000, tol=1e-6, raise_on_noconv=True, precision="fp64"):
         """Lock-step GMRES for the rows of RHS [nrhs, n]; returns (X, [info dicts])."""
         Bm = np.ascontiguousarray(np.atleast_2d(RHS), dtype=np.complex128)
         nr = Bm.shape[0]
         X = np.zeros_like(Bm)
-        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), 0)
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), 0, 0, _prec(precision))
         infos = (_lib.SolveInfo * nr)()
         rc = lib().ifs_solve_batch(self._h, nr, Bm.ctypes.data, X.ctypes.data, self.n, ctypes.byref(op), infos)
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
@@ -169,8 +179,10 @@ class MultiSphereSolver:
         return X, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
                         rel_residual_flat=i.rel_residual_flat, seconds=i.seconds) for i in infos]
 
-    def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False, verbose=False):
-        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)), int(bool(verbose)))
+    def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False, verbose=False,
+                     precision="fp64"):
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)), int(bool(verbose)),
+                            _prec(precision))
         inf = _lib.SolveInfo()
         rc = lib().ifs_solve_device(self._h, ctypes.c_void_p(rhs_ptr), ctypes.c_void_p(x_ptr), self.n,
                                     ctypes.byref(op), ctypes.byref(inf))
@@ -178,6 +190,10 @@ class MultiSphereSolver:
             check(rc, "ifs_solve_device")
         return dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
                     rel_residual_flat=inf.rel_residual_flat, seconds=inf.seconds, status=rc)
+
+    def set_precision(self, precision):
+        """'fp64' or 'mixed' for this context's matvecs from now on."""
+        check(lib().ifs_set_precision(self._h, _prec(precision)), "ifs_set_precision")
 
     # -- instrumentation ------------------------------------------------------
     def set_stream(self, stream_ptr):
