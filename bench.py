@@ -25,6 +25,14 @@ FP64_PEAK_TFLOPS = 78.6
 METRIC = "MLFMM matvecs/s & solves/s at rel. residual 1e-6; p50/p99 solve latency"
 
 
+def snm_flat(N, ns):
+    """s_nm = sqrt(n(n+1)(2n+1)(n-m)!/(n+m)!) over the flat vector (a and b blocks, ns spheres)."""
+    from math import lgamma
+    s = np.array([np.exp(0.5 * (np.log(n * (n + 1) * (2 * n + 1)) + lgamma(n - m + 1) - lgamma(n + m + 1)))
+                  for n in range(1, N + 1) for m in range(-n, n + 1)])
+    return np.tile(np.concatenate([s, s]), ns)
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -57,9 +65,10 @@ def cpu_baseline(cluster, npairs, threads, affinity):
                       f"threads, extrapolated to one direct matvec over all {ns * (ns - 1)} ordered pairs"}
 
 
-def solve_latency(seeds, x0, max_iter, log, n=None):
+def solve_latency(seeds, x0, max_iter, log, n=None, precision="fp64"):
     """End-to-end request latency on fresh contexts: create (octree, Mie, plan) + first
-    matvec's table generation + GMRES(50) to 1e-6 from the single-sphere solution."""
+    matvec's table generation + GMRES(50) to 1e-6 from the single-sphere solution
+    (precision 'mixed': FP32 box products in the Arnoldi steps, FP64 true residuals)."""
     import torch
     from itofoggysim_amd import MultiSphereSolver, config_cluster
     runs = []
@@ -74,14 +83,14 @@ def solve_latency(seeds, x0, max_iter, log, n=None):
         xs = rhs.clone() if x0 == "single" else torch.zeros_like(rhs)
         torch.cuda.synchronize()
         info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=max_iter, tol=1e-6,
-                              use_x0=x0 == "single", verbose=False)
+                              use_x0=x0 == "single", verbose=False, precision=precision)
         total = time.perf_counter() - t0
         s.close()
         del rhs, xs
         runs.append({"seed": seed, "create_s": round(t_build, 3), "solve_s": round(info["seconds"], 3),
                      "total_s": round(total, 3), "iterations": info["iterations"], "converged": info["converged"],
                      "rel_residual": info["rel_residual"], "rel_residual_flat": info["rel_residual_flat"]})
-        print(f"# latency seed {seed}: {total:.1f} s end to end ({info['iterations']} matvecs, "
+        print(f"# latency {precision} seed {seed}: {total:.1f} s end to end ({info['iterations']} matvecs, "
               f"residual {info['rel_residual']:.2e})", file=log, flush=True)
     tot = sorted(r["total_s"] for r in runs)
     # p50 / p99 by linear interpolation over the samples
@@ -108,7 +117,7 @@ def main():
                     help="GMRES initial guess: the single-sphere solution T_j p_j, i.e. the right-hand side "
                          "(lib_mie_multi_sphere.f90:134-138), or zero")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
-    ap.add_argument("--latency-seeds", type=int, default=3,
+    ap.add_argument("--latency-seeds", type=int, default=2,
                     help="clusters (seeds 2000, 2001, ...) for the end-to-end solve latency; 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE counters)")
@@ -168,7 +177,35 @@ def main():
         s.calculate_vector_b_device(x.data_ptr(), b.data_ptr())
         prof = s.profile_read()
         s.profile(False)
+        # the same matvec with FP32 matrix-core box translations (ifs_set_precision)
+        s.set_precision("mixed")
+        b32 = torch.empty_like(x)
+        s.calculate_vector_b_device(x.data_ptr(), b32.data_ptr())
+        st.synchronize()
+        e0.record(st)
+        for i in range(args.steps):
+            s.calculate_vector_b_device(x.data_ptr(), b32.data_ptr())
+        e1.record(st)
+        st.synchronize()
+        ms32 = e0.elapsed_time(e1) / args.steps
+        s.profile(True)
+        s.calculate_vector_b_device(x.data_ptr(), b32.data_ptr())
+        prof32 = s.profile_read()
+        s.profile(False)
+        # accuracy of the mixed matvec on an input random in the normalised basis (x = x_hat / s_nm;
+        # the timed x is random in the flat basis, whose normalised entries span 1e+-80)
+        snm = torch.from_numpy(snm_flat(s.n_global, s.n_spheres)).to("cuda")
+        xe = torch.randn(s.n, dtype=torch.complex128, generator=gen).to("cuda") / snm
+        be32 = torch.empty_like(x)
+        be64 = torch.empty_like(x)
+        s.calculate_vector_b_device(xe.data_ptr(), be32.data_ptr())
+        s.set_precision("fp64")
+        s.calculate_vector_b_device(xe.data_ptr(), be64.data_ptr())
+        st.synchronize()
+        print(f"# mixed: {ms32:.1f} ms/step", file=sys.stderr, flush=True)
     finite = bool(torch.isfinite(b).all().item())
+    # mixed vs FP64 matvec, relative in the normalised basis
+    mixed_err = float(torch.linalg.norm((be32 - be64) * snm) / torch.linalg.norm((be64 - xe) * snm))
 
     kern = {k: v for k, v in prof.items() if v[2] > 0}
     dom = max(kern, key=lambda k: kern[k][0])
@@ -220,6 +257,14 @@ def main():
         "kernels_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) for k, v in kern.items() if v[0] > 0},
         "finite": finite,
     }
+    out["mixed"] = {
+        "value": round(1000.0 / ms32, 6), "unit": "matvecs/s", "ms_per_step": round(ms32, 3),
+        "speedup_vs_fp64": round(ms / ms32, 3),
+        "rel_error_vs_fp64_normalised": mixed_err,
+        "kernels_ms": {k: round(v[0], 2) for k, v in prof32.items()},
+        "kernels_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) for k, v in prof32.items() if v[2] > 0 and v[0] > 0},
+        "note": "M2M/M2L/L2L as FP32 matrix-core products (FP32 accumulation) on the FP64 tables; "
+                "sphere-level translations, T-apply and vectors FP64; FP32 peak 157.3 TFLOP/s"}
     if not args.no_solve:
         rhs = torch.from_numpy(s.get_vector_b()).to("cuda")
         # the single-sphere solution of (I + T A) x = T p is x_j = T_j p_j, the right-hand side itself
@@ -232,15 +277,27 @@ def main():
         info["note"] = "tables generated in the warm-up matvec (steady-state solve)"
         out["solve"] = info
         out["solves_per_s_steady"] = 1.0 / info["seconds"] if info["converged"] else None
+        xs = rhs.clone() if args.x0 == "single" else torch.zeros_like(rhs)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(st):
+            info32 = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=args.solve_max_iter,
+                                    tol=1e-6, use_x0=args.x0 == "single", verbose=True, precision="mixed")
+        out["solve_mixed"] = info32
+        out["solves_per_s_steady_mixed"] = 1.0 / info32["seconds"] if info32["converged"] else None
         del rhs, xs
     s.close()
     if not args.no_solve and args.latency_seeds > 0:
         torch.cuda.synchronize()
-        lat = solve_latency([2000 + i for i in range(args.latency_seeds)], args.x0, args.solve_max_iter, sys.stderr,
-                            n=args.n if args.n != 2000 else None)
+        seeds = [2000 + i for i in range(args.latency_seeds)]
+        nn = args.n if args.n != 2000 else None
+        lat = solve_latency(seeds, args.x0, args.solve_max_iter, sys.stderr, n=nn)
         out["solve_latency_s"] = {k: lat[k] for k in ("p50_s", "p99_s", "samples")}
         out["solves_per_s"] = lat["solves_per_s"]
         out["solve_latency_runs"] = lat["runs"]
+        lat32 = solve_latency(seeds, args.x0, args.solve_max_iter, sys.stderr, n=nn, precision="mixed")
+        out["mixed"]["solve_latency_s"] = {k: lat32[k] for k in ("p50_s", "p99_s", "samples")}
+        out["mixed"]["solves_per_s"] = lat32["solves_per_s"]
+        out["mixed"]["solve_latency_runs"] = lat32["runs"]
     if not args.no_cpu:
         # every core of the host share this process runs on: OMP_NUM_THREADS where the host
         # sets it (the CPU share of one GPU on a shared node), else all of nproc

@@ -704,6 +704,9 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
     class_col_flops(T0, ffwd, fax, fback);
     const int L = std::max(T0.Ls, T0.Lt);
     const int64_t nbox = 2LL * T0.Lt * (T0.Lt + 2);
+    // the FP32 products take Wigner tables regenerated per matvec in FP32 (half the bytes to
+    // store and to read); tables kept across matvecs stay FP64 and are converted at the load
+    const int ftab = prec == 1 && !B.d_persist ? 1 : 0;
     int cur_ax = -1;                                        // coaxial key in the shared workspace
     std::vector<int> owner(std::max(B.n_dslots, 1), -1);    // Wigner key in each shared slot
     auto need_d = [&](const ClassBatch::Piece &P) {
@@ -716,8 +719,13 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
             if (owner[slot] == key) return;
             owner[slot] = key;
         }
-        Scope s(c, "class_gen_wigner");
-        launch_gen_wigner_fold(dk + key, 1, (T0.K + 1) * (L + 1), wd + P.g.off_d, c->stream);
+        if (ftab) {
+            Scope s(c, "class_gen_wigner_f32");
+            launch_gen_wigner_fold_f32(dk + key, 1, (T0.K + 1) * (L + 1), (float *)wd + P.g.off_d, c->stream);
+        } else {
+            Scope s(c, "class_gen_wigner");
+            launch_gen_wigner_fold(dk + key, 1, (T0.K + 1) * (L + 1), wd + P.g.off_d, c->stream);
+        }
     };
     auto need_ax = [&](const ClassBatch::Piece &P) {
         const int key = B.axkey[P.gen];
@@ -741,18 +749,18 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
         for (auto &P : ch.runs) {
             need_d(P);
             Scope s(c, pn_fwd, ffwd * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, prec, c->stream);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, prec, ftab, c->stream);
         }
         for (auto &P : ch.segs) {
             need_ax(P);
             Scope s(c, pn_ax, fax * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, prec, c->stream);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, prec, ftab, c->stream);
         }
         for (size_t r = ch.runs.size(); r-- > 0;) {
             auto &P = ch.runs[r];
             need_d(P);
             Scope s(c, pn_back, fback * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, prec, c->stream);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, prec, ftab, c->stream);
         }
         {
             Scope s(c, "class_scatter");

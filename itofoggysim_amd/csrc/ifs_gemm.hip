@@ -94,13 +94,13 @@ constexpr int FNF2 = 64;
 constexpr int FNB = 64;
 constexpr int FK = 16;
 
-template <typename TC, int RM, int MINB>
+template <typename TC, typename TW, int RM, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__restrict__ desc,
                                                        const int64_t *__restrict__ src_off,
                                                        const int32_t *__restrict__ pidx,
                                                        const double2 *__restrict__ phase,
                                                        const double2 *__restrict__ src_pool,
-                                                       const double *__restrict__ wd, TC *__restrict__ wy) {
+                                                       const TW *__restrict__ wd, TC *__restrict__ wy) {
     typedef typename Mx<TC>::v4 acc4;
     const GroupDesc C = *desc;
     const int nu = blockIdx.z + 1;
@@ -123,8 +123,8 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
     __shared__ TC smem[AS + BS > XS ? AS + BS : XS];
     TC(*As)[2 * RM + RPAD] = reinterpret_cast<TC(*)[2 * RM + RPAD]>(smem);
     TC(*Bs)[FK][FNF2 + RPAD] = reinterpret_cast<TC(*)[FK][FNF2 + RPAD]>(smem + AS);
-    const double *E = wd + C.off_d + fold_off(nu, K);
-    const double *O = E + (int64_t)M * KI;
+    const TW *E = wd + C.off_d + fold_off(nu, K);
+    const TW *O = E + (int64_t)M * KI;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -149,15 +149,15 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
     const int tt = jt % C.ncols, rr = jt / C.ncols;  // translation, right-hand side
     const double2 *xs = src_pool + rr * C.src_stride + src_off[C.col_begin + tt] + fidx(nu, 0);  // xs[m]
     const double2 *ps = phase + (int64_t)pidx[C.col_begin + tt] * (2 * C.PL + 1) + C.PL;        // e^{i m phi}
-    double ra[NE];
+    TW ra[NE];
     double2 rap[1], rbp[1], ram[1], rbm[1], rph[1];
     auto load = [&](int k0, auto tail) {
         int gq = k0 + q;
         if constexpr (decltype(tail)::value) gq = min(gq, KI - 1);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            double v = (e < NE / 2 ? E : O)[roff[e % (NE / 2)] + gq];
-            if constexpr (decltype(tail)::value) v = k0 + q < KI ? v : 0.0;
+            TW v = (e < NE / 2 ? E : O)[roff[e % (NE / 2)] + gq];
+            if constexpr (decltype(tail)::value) v = k0 + q < KI ? v : (TW)0;
             ra[e] = v;
         }
         rap[0] = xs[gq];
@@ -283,11 +283,11 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_fwd(const GroupDesc *__
         }
 }
 
-template <typename TC, int FM, int MINB>
+template <typename TC, typename TW, int FM, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *__restrict__ desc,
                                                         const int32_t *__restrict__ pidx,
                                                         const double2 *__restrict__ phase,
-                                                        const double *__restrict__ wd, const TC *__restrict__ wz,
+                                                        const TW *__restrict__ wd, const TC *__restrict__ wz,
                                                         double2 *__restrict__ W) {
     typedef typename Mx<TC>::v4 acc4;
     const GroupDesc C = *desc;
@@ -305,8 +305,8 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
     if (row0 >= M) return;
     __shared__ TC As[FK][2 * FM + RPAD];     // columns 0..FM-1: E^T rows, FM..2FM-1: O^T rows
     __shared__ TC Bs[2][FK][FNB + RPAD];     // s, t
-    const double *E = wd + C.off_d + fold_off(n, K);  // E[k][m], KI x M
-    const double *O = E + (int64_t)KI * M;
+    const TW *E = wd + C.off_d + fold_off(n, K);  // E[k][m], KI x M
+    const TW *O = E + (int64_t)KI * M;
     const TC *Z = wz + rows_off(n, K) * (int64_t)NCY + 4 * (int64_t)C.ycol0 * C.nrhs;  // row kk + k
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -325,22 +325,22 @@ __global__ __launch_bounds__(256, MINB) void k_class_rot_back(const GroupDesc *_
         }
     // raw loads only: the s/t combination happens at the LDS store, so that the
     // prefetch of the next step stays in flight while the matrix cores work
-    double ra[NEA];
+    TW ra[NEA];
     TC rzp[FNB / 16], rzm[FNB / 16];
     // Loads without bounds branches.  A thread's table row m (tid & 63 of E or O) and
     // Z column (tid & 63) are clamped into the operands: rows m >= M and columns
     // >= NC only feed outputs that are never stored.  Inner entries k >= KI occur in
     // the last step only, which clamps them and zeroes their table entries; z_{-0}
     // is zeroed at the LDS store of the first step.
-    const double *pa = ((tid & (2 * FM - 1)) < FM ? E : O) + min(row0 + (tid & (FM - 1)), M - 1);
+    const TW *pa = ((tid & (2 * FM - 1)) < FM ? E : O) + min(row0 + (tid & (FM - 1)), M - 1);
     const TC *pz = Z + min(col0 + (tid & (FNB - 1)), NC - 1);
     auto load = [&](int k0, auto tail) {
 #pragma unroll
         for (int e = 0; e < NEA; ++e) {
             const int gk = k0 + tid / (2 * FM) + (256 / (2 * FM)) * e;  // 16 inner x 2 FM rows (E, O), coalesced along m
             if constexpr (decltype(tail)::value) {
-                const double v = pa[min(gk, KI - 1) * M];
-                ra[e] = gk < KI ? v : 0.0;
+                const TW v = pa[min(gk, KI - 1) * M];
+                ra[e] = gk < KI ? v : (TW)0;
             } else {
                 ra[e] = pa[gk * M];
             }
@@ -657,16 +657,16 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterSeg *__restrict__ 
     }
 }
 
-template <typename TC>
+template <typename TC, typename TW>
 static void launch_class_t(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
-                           const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax,
+                           const double2 *phase, const double2 *src_pool, const TW *wd, const double2 *wax,
                            TC *wy, TC *wz, int stage, hipStream_t s) {
     const int NC = 4 * h.ncols * h.nrhs;
     const int K = h.K;
     if (stage == 0) {
         int mt = ((h.Ls < K ? h.Ls : K) + 1 + FMF - 1) / FMF;
         dim3 grid((NC + FNF2 - 1) / FNF2, mt, h.Ls);
-        hipLaunchKernelGGL((k_class_rot_fwd<TC, FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool,
+        hipLaunchKernelGGL((k_class_rot_fwd<TC, TW, FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool,
                            wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
@@ -682,19 +682,23 @@ static void launch_class_t(const GroupDesc &h, const GroupDesc *d_desc, const in
         int mt = (h.Lt + 1 + FMB - 1) / FMB;
         dim3 grid((NC + FNB - 1) / FNB, mt, h.Lt);
         // W (FP64 columns) aliases the Y workspace, free once the coaxial shifts are done
-        hipLaunchKernelGGL((k_class_rot_back<TC, FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd,
+        hipLaunchKernelGGL((k_class_rot_back<TC, TW, FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd,
                            (const TC *)wz, (double2 *)wy);
     }
 }
 
 void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
                   const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
-                  double *wz, int stage, int prec, hipStream_t s) {
+                  double *wz, int stage, int prec, int ftab, hipStream_t s) {
     if (h.ncols <= 0 || h.nrhs <= 0) return;
-    if (prec == 1)
-        launch_class_t<float>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, (float *)wy, (float *)wz, stage, s);
+    if (prec == 1 && ftab)
+        launch_class_t<float, float>(h, d_desc, src_off, pidx, phase, src_pool, (const float *)wd, wax, (float *)wy,
+                                     (float *)wz, stage, s);
+    else if (prec == 1)
+        launch_class_t<float, double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, (float *)wy, (float *)wz,
+                                      stage, s);
     else
-        launch_class_t<double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, wy, wz, stage, s);
+        launch_class_t<double, double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, wy, wz, stage, s);
 }
 
 void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,

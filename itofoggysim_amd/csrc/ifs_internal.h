@@ -167,6 +167,8 @@ void launch_gen_axial_gqb(const Task *tasks, int ntasks, const GqbChunk &g, cons
 
 // Kernel launchers (ifs_kernels.hip).  All enqueue on `s`.
 void launch_gen_wigner_fold(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s);
+// Same tables stored in FP32 (the recurrence runs in FP64).
+void launch_gen_wigner_fold_f32(const Task *tasks, int ntasks, int maxpairs, float *wd, hipStream_t s);
 void launch_gen_sect(const Task *tasks, int ntasks, int max_nmax, double2 *wsect, hipStream_t s);
 void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect, double2 *wax,
                       hipStream_t s);
@@ -186,14 +188,15 @@ void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblo
                      const double2 *wz, double2 *dst_pool, hipStream_t s);
 
 // Precision of the class products (prec): 0 FP64 throughout; 1 FP32 products with FP32
-// accumulation (tables and pools stay FP64 and are converted where staged into LDS;
-// Y and Z hold FP32, the back-rotated columns W FP64).
+// accumulation (pools and coaxial tables stay FP64 and are converted where staged into
+// LDS; Y and Z hold FP32, the back-rotated columns W FP64).  ftab: the Wigner tables at
+// wd are FP32 (prec 1 only; tables regenerated per matvec are then generated in FP32).
 // stage 0: forward rotation of a run (source gather with exp(i m phi)) -> Y
 // stage 1: coaxial shift of a segment Y -> Z
 // stage 2: back rotation of a run (exp(-i m phi)) Z -> W (aliases Y)
 void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
                   const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
-                  double *wz, int stage, int prec, hipStream_t s);
+                  double *wz, int stage, int prec, int ftab, hipStream_t s);
 // dst_pool[r dst_stride + seg.dst + e] += sum_{i in [beg,end)} W[r ncy + cols[i]][e],
 // e < nbox = 2 Lt(Lt+2), r < nrhs
 void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,

@@ -66,7 +66,8 @@ __device__ __forceinline__ double coef_b(int n, int m) {
 // ------------------------------------------------------------------------
 // Folded tables (layout at fold_off): threads over (k, m), k = 0..K, m = 0..L; each
 // runs the recurrences of d_{k,m} and d_{k,-m} side by side from n0 = max(k, m).
-__global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__restrict__ wd) {
+template <typename TW>
+__global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, TW *__restrict__ wd) {
     const Task T = tasks[blockIdx.y];
     const int K = T.K;
     const int L = T.Ls > T.Lt ? T.Ls : T.Lt;
@@ -85,15 +86,15 @@ __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__rest
     // 2 (min(n,K)+1)(n+1) + k past that of degree n, so a pointer is carried from
     // degree to degree (32-bit increments: box orders below 32000).
     const int nfirst = n0 == 0 ? 1 : n0;
-    double *pe = wd + T.off_d + fold_off(nfirst, K) + (int64_t)k * (nfirst + 1) + m;
+    TW *pe = wd + T.off_d + fold_off(nfirst, K) + (int64_t)k * (nfirst + 1) + m;
     const bool m0 = m == 0, k0 = k == 0;
     auto put = [&](int n, double dp, double dm) {
         const int kk = n < K ? n : K;
         const double e = m0 ? 2.0 * dp : dp + sgm * dm;
         const double od = (m0 || k0) ? 0.0 : dp - sgm * dm;
         const int ob = (kk + 1) * (n + 1);
-        pe[0] = e;
-        pe[ob] = od;
+        pe[0] = (TW)e;
+        pe[ob] = (TW)od;
         pe += 2 * ob + k;
     };
     double p1, p2, q1, q2;  // chains d_{k,m} and d_{k,-m}: values at n-1 and n-2
@@ -138,7 +139,13 @@ __global__ void k_gen_wigner_fold(const Task *__restrict__ tasks, double *__rest
 void launch_gen_wigner_fold(const Task *tasks, int ntasks, int maxpairs, double *wd, hipStream_t s) {
     if (ntasks <= 0) return;
     dim3 grid((maxpairs + 255) / 256, ntasks);
-    hipLaunchKernelGGL(k_gen_wigner_fold, grid, dim3(256), 0, s, tasks, wd);
+    hipLaunchKernelGGL(k_gen_wigner_fold<double>, grid, dim3(256), 0, s, tasks, wd);
+}
+
+void launch_gen_wigner_fold_f32(const Task *tasks, int ntasks, int maxpairs, float *wd, hipStream_t s) {
+    if (ntasks <= 0) return;
+    dim3 grid((maxpairs + 255) / 256, ntasks);
+    hipLaunchKernelGGL(k_gen_wigner_fold<float>, grid, dim3(256), 0, s, tasks, wd);
 }
 
 // ------------------------------------------------------------------------
