@@ -76,3 +76,31 @@ def test_i4_bit_exact(L):
             ref = [v for v in ref[:c] if v in present]
             assert cnt[q] == len(ref) and list(out[q, :cnt[q]]) == ref, (l, b)
             assert np.all(out[q, cnt[q]:] == -1)
+
+
+@pytest.mark.parametrize("s", [10, 32])
+def test_build_vs_oracle_restatement(L, s):
+    """The GPU build against the oracle's independent restatement of libtree
+    (oracle/fmm_ref.py: floor(x 2^l) bit interleave, l_max by counting, I4 as the
+    children of the parent's neighbourhood minus the own neighbourhood), on the same
+    clouds: Morton keys at level 21, l_max and the I4 lists of the occupied boxes."""
+    from oracle import fmm_ref
+    for name, pts in _clouds():
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        gu, perm, lv = _gpu_tree(L, pts, s)
+        scaled, _, _ = fmm_ref.scale_points(pts)
+        keys = np.array([fmm_ref.morton(p, 21) for p in scaled], dtype=np.int64)
+        assert np.array_equal(np.sort(keys), gu), name
+        lmax = max(fmm_ref.level_max(scaled, s), 2)
+        assert lv[1] == lmax, (name, lv, lmax)
+        l = int(lv[1])
+        boxes = np.unique(gu >> (3 * (21 - l))).astype(np.int64)
+        out = np.empty(189 * len(boxes), dtype=np.int64)
+        cnt = np.empty(len(boxes), dtype=np.int32)
+        assert L.ifs_tree_i4_gpu(boxes.ctypes.data, len(boxes), l, out.ctypes.data, cnt.ctypes.data) == 0
+        out = out.reshape(-1, 189)
+        present = set(boxes.tolist())
+        for q, b in enumerate(boxes):
+            ref = [v for v in fmm_ref.i4(int(b), l) if v in present]
+            assert list(out[q, :cnt[q]]) == ref, (name, l, b)
+        print(f"{name}: n={len(pts)} s={s} l_max={l}, {len(boxes)} leaf boxes: GPU = oracle restatement")
