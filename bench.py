@@ -117,6 +117,12 @@ def main():
                     help="GMRES initial guess: the single-sphere solution T_j p_j, i.e. the right-hand side "
                          "(lib_mie_multi_sphere.f90:134-138), or zero")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="serving workload: Poisson arrivals of this many requests (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--serve-nmax", type=int, default=500,
+                    help="largest cluster of the serving workload (the stated workload goes to 20000)")
+    ap.add_argument("--serve-dirs", type=int, default=8, help="most incidence directions of a serving request")
     ap.add_argument("--latency-seeds", type=int, default=2,
                     help="clusters (seeds 2000, 2001, ...) for the end-to-end solve latency; 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -298,6 +304,33 @@ def main():
         out["mixed"]["solve_latency_s"] = {k: lat32[k] for k in ("p50_s", "p99_s", "samples")}
         out["mixed"]["solves_per_s"] = lat32["solves_per_s"]
         out["mixed"]["solve_latency_runs"] = lat32["runs"]
+    if args.serve_requests > 0:
+        from itofoggysim_amd import Server, poisson_workload, run_workload
+        torch.cuda.synchronize()
+        reqs = poisson_workload(args.serve_requests, args.serve_rate, seed=5, n_min=10, n_max=args.serve_nmax,
+                                max_dirs=args.serve_dirs)
+        srv = Server(slots=8, hbm_budget_gb=200.0)
+        t0 = time.perf_counter()
+        res, stt = run_workload(srv, reqs, tol=1e-6)
+        wall = time.perf_counter() - t0
+        srv.close()
+        ok = all(info["status"] == 0 for _, info in res)
+        nsolves = sum(len(r["directions"]) for r in reqs)
+        out["serving"] = {
+            "workload": f"{len(reqs)} requests, Poisson arrivals at {args.serve_rate} /s, N log-uniform in "
+                        f"[10, {args.serve_nmax}] (20-sphere benchmark radius law and number density), lambda "
+                        f"U[400,1550] nm with n = 1.3199 + 3500 nm^2/lambda^2 + 1e-8 i, 1-{args.serve_dirs} directions, "
+                        f"half plain-FP64 / half mixed, GMRES(30) to 1e-6, 8 slots, 200 GB budget, shortest estimated request first with aging",
+            "all_ok": ok, "requests": len(reqs), "solves": nsolves, "wall_s": round(wall, 3),
+            "p50_latency_s": round(float(stt["p50_s"]), 4), "p99_latency_s": round(float(stt["p99_s"]), 4),
+            "requests_per_s": round(len(reqs) / wall, 4), "solves_per_s": round(nsolves / wall, 4),
+            "peak_concurrent": stt["peak_concurrent"], "peak_admitted_gb": round(stt["peak_admitted_bytes"] / 2**30, 2),
+            "per_request": [{"n": len(r["cluster"]["radii"]), "lambda_um": round(r["cluster"]["wavelength"], 4),
+                             "ndir": len(r["directions"]), "precision": r["precision"],
+                             "latency_s": round(info["latency_s"], 3), "queue_s": round(info["queue_s"], 3),
+                             "footprint_gb": round(info["footprint_bytes"] / 2**30, 3)} for r, (_, info) in zip(reqs, res)]}
+        print(f"# serving: {len(reqs)} requests in {wall:.1f} s, p50 {stt['p50_s']:.2f} s, p99 {stt['p99_s']:.2f} s",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         # every core of the host share this process runs on: OMP_NUM_THREADS where the host
         # sets it (the CPU share of one GPU on a shared node), else all of nproc

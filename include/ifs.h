@@ -242,12 +242,16 @@ int ifs_tree_i4_gpu(const int64_t *boxes, int64_t nb, int32_t level, int64_t *ou
 /* Serving: continuous batching of scattering requests on one GPU.  A request is a
  * cluster with its parameters and ndir incidence directions (alpha[i], beta[i]); its
  * result is {C_sca, C_ext, C_abs} per direction, solved to the relative residual tol
- * with lock-step GMRES(restart).  `slots` worker threads each run one request at a
- * time on a context with its own stream, so admitted requests overlap on the device;
- * a queued request is admitted when a slot is free and its estimated HBM footprint
- * fits what the running ones leave of hbm_budget_gb (or nothing else runs).  Inputs
- * are copied at submit.  No reference counterpart (the reference solves one cluster
- * per process). */
+ * with lock-step GMRES(restart) in the request's precision (ifs_params.precision).
+ * `slots` worker threads each run one request at a time on a context with its own
+ * stream, so admitted requests overlap on the device.  At submit the request's device
+ * memory is planned on the host (vectors and Krylov basis, MLFMM pools from an octree
+ * built with the ifs_tree_* functions, translation workspace, kept tables), and its
+ * context is created with that workspace and table cache (ifs_params.workspace_mb and
+ * cache_mb are overridden), so it stays inside its footprint; a queued request is
+ * admitted when a slot is free and its footprint fits what the running ones leave of
+ * hbm_budget_gb (or nothing else runs).  Inputs are copied at submit.  No reference
+ * counterpart (the reference solves one cluster per process). */
 typedef struct ifs_server ifs_server;
 typedef struct {
     int32_t status;          /* IFS_OK or the request's error */

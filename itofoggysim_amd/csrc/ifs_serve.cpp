@@ -35,6 +35,7 @@ struct Req {
     double tol = 1e-6;
     int restart = 30, max_iter = 1000;
     size_t footprint = 0;
+    double est_s = 0.0;  // estimated device time (admission order)
     clk::time_point t_submit, t_start, t_done;
     // result
     int status = 0;
@@ -52,6 +53,7 @@ struct ifs_server {
     std::map<int64_t, Req> reqs;
     std::vector<std::thread> workers;
     size_t budget = 0, admitted = 0, peak_admitted = 0;
+    int slots = 1;
     int running = 0, peak_running = 0;
     int64_t next_id = 1;
     bool stop = false;
@@ -62,9 +64,41 @@ struct ifs_server {
 
 namespace {
 
-// Device bytes a request needs: vectors, Krylov basis of the lock-step GMRES and
-// the direct-path translation tables (kept for the whole request).
-size_t estimate_bytes(const Req &r, size_t *tables = nullptr) {
+// Device memory of a request, planned on the host before admission, and the limits its
+// context is created with so that it stays inside the plan (ifs_create would otherwise
+// size its workspace and table cache from the free memory of the whole device):
+//   vectors    right-hand sides, iterates, the lock-step GMRES basis, compact copies;
+//   pools      MLFMM multipole and local expansions of every box (octree built on the
+//              host with the library's own functions: scaling, Morton keys, l_max);
+//   workspace  the translation workspace (ifs_params.workspace_mb), at most a quarter
+//              of a slot's share of the budget and at most 8 GiB;
+//   tables     translation tables kept across matvecs (ifs_params.cache_mb): the
+//              direct path's tables or the MLFMM's M2L coaxial tables, when they fit
+//              the rest of the slot's share; otherwise regenerated in every matvec;
+//   fixed      per-context buffers and the quadrature workspaces.
+struct Plan {
+    size_t vectors = 0, pools = 0, workspace = 0, tables = 0, fixed = 0;
+    double flops = 0.0;  // translation flops of one matvec per right-hand side (about 21 L^3 each)
+    size_t total() const { return vectors + pools + workspace + tables + fixed; }
+};
+
+int box_L(double k, double edge, double d0, int cap) {
+    const double kD = k * edge;
+    int L = (int)std::ceil(kD + 1.8 * std::pow(d0, 2.0 / 3.0) * std::cbrt(kD));
+    if (L < 1) L = 1;
+    if (cap > 0 && L > cap) L = cap;
+    return L;
+}
+
+// sum_{q=1..K} 2 (L+1-q)^2 + (L)^2 complex per coaxial table of orders (L, L), both P and Q
+double coax_table_bytes(int L) {
+    double e = (double)L * L;
+    for (int q = 1; q <= L; ++q) e += 2.0 * (L + 1.0 - q) * (L + 1.0 - q);
+    return 2.0 * 16.0 * e;
+}
+
+Plan plan_request(Req &r, size_t budget, int slots) {
+    Plan P;
     const int ns = (int)r.radii.size();
     int N = r.prm.n_global;
     const double k = r.prm.k;
@@ -76,17 +110,59 @@ size_t estimate_bytes(const Req &r, size_t *tables = nullptr) {
     if (N <= 0) N = *std::max_element(L.begin(), L.end());
     const double n = 2.0 * ns * N * (N + 2);
     const int nd = (int)r.alpha.size();
-    double bytes = 16.0 * n * ((r.restart + 1) * nd + 6.0 * nd + 4.0);
-    double tab = 0.0;
-    for (int a = 0; a < ns; ++a)
-        for (int b = 0; b < ns; ++b) {
-            if (a == b) continue;
-            double Ls = L[a], Lt = L[b], K = std::min(Ls, Lt), Lm = std::max(Ls, Lt);
-            tab += 8.0 * 2.0 * (K + 1) * (Lm + 1) * (Lm + 2) / 2.0 + 16.0 * 2.0 * (2 * K + 1) * Ls * Lt +
-                   16.0 * 2.0 * (2 * K + 1) * (Ls + Lt);
+    P.vectors = (size_t)(16.0 * n * ((r.restart + 1) * nd + 6.0 * nd + 4.0));
+    const size_t share = budget / (size_t)std::max(slots, 1);
+    const size_t ws_cap = std::min<size_t>(std::max<size_t>(share / 4, (size_t)256 << 20), (size_t)8 << 30);
+    double table_need = 0.0, ws_need = 0.0;
+    if (r.prm.use_fmm) {
+        std::vector<double> scaled(3 * ns), xmd(4);
+        std::vector<int64_t> u(ns);
+        ifs_tree_scale(r.centres.data(), ns, scaled.data(), xmd.data());
+        ifs_tree_universal_index(scaled.data(), ns, 21, u.data());
+        std::sort(u.begin(), u.end());
+        const int s = r.prm.leaf_size > 0 ? r.prm.leaf_size : 10;
+        const int lmax = std::max(2, ifs_tree_level_max(u.data(), ns, s));
+        const double d0 = r.prm.fmm_d0 > 0.0 ? r.prm.fmm_d0 : 6.0;
+        double pools = 0.0;
+        for (int l = 2; l <= lmax; ++l) {
+            std::vector<int64_t> b(u.size());
+            for (int i = 0; i < ns; ++i) b[i] = u[i] >> (3 * (21 - l));
+            b.erase(std::unique(b.begin(), b.end()), b.end());
+            const int Lb = box_L(k, xmd[3] / std::ldexp(1.0, l), d0, r.prm.fmm_lmax_cap);
+            pools += 2.0 * (double)b.size() * 2.0 * Lb * (Lb + 2.0) * 16.0 * nd;
+            P.flops += (double)b.size() * 100.0 * 21.0 * Lb * Lb * (double)Lb;  // ~100 M2L per box
+            // M2L: at most 15 distinct offset lengths per level, one coaxial table each
+            table_need += 15.0 * coax_table_bytes(Lb);
+            // class columns: up to 189 M2L per box, Y/W and Z of 4 L(L+2) doubles each
+            ws_need += (double)b.size() * 189.0 * 2.0 * 4.0 * Lb * (Lb + 2.0) * 8.0 * nd;
+            // sphere <-> box quadrature tables of one chunk
+            ws_need = std::max(ws_need, (double)ns * 2.0 * 16.0 * Lb * 51.0 * 51.0);
         }
-    if (tables) *tables = (size_t)tab;
-    return (size_t)(bytes + tab) + ((size_t)64 << 20);
+        P.pools = (size_t)pools;
+        double l3 = 0.0;
+        for (int j = 0; j < ns; ++j) l3 += (double)L[j] * L[j] * L[j];
+        P.flops += 27.0 * std::min(ns, 2 * s) * 21.0 * l3;  // near field: about 27 leaves of up to 2 s spheres
+        P.fixed = ((size_t)1 << 30) + ((size_t)256 << 20);  // quadrature workspaces and tables of the box levels
+    } else {
+        for (int a = 0; a < ns; ++a)
+            for (int b = 0; b < ns; ++b) {
+                if (a == b) continue;
+                double Ls = L[a], Lt = L[b], K = std::min(Ls, Lt), Lm = std::max(Ls, Lt);
+                table_need += 8.0 * 2.0 * (K + 1) * (Lm + 1) * (Lm + 2) / 2.0 + 16.0 * 2.0 * (2 * K + 1) * Ls * Lt +
+                              16.0 * 2.0 * (2 * K + 1) * (Ls + Lt);
+                P.flops += 21.0 * Lm * Lm * Lm;
+            }
+        ws_need = table_need * nd;
+        P.fixed = (size_t)64 << 20;
+    }
+    P.workspace = (size_t)std::min<double>((double)ws_cap, ws_need + (64 << 20));
+    // tables kept only when they fit what the slot's share leaves
+    const size_t used = P.vectors + P.pools + P.workspace + P.fixed;
+    if (table_need > 0.0 && used + (size_t)table_need <= share) P.tables = (size_t)table_need;
+    r.prm.workspace_mb = (int32_t)std::max<size_t>(1, P.workspace >> 20);
+    r.prm.cache_mb = P.tables > 0 ? (int32_t)((P.tables >> 20) + 1) : -1;
+    if (P.tables > 0) P.tables = (size_t)r.prm.cache_mb << 20;
+    return P;
 }
 
 void run_request(Req &r) {
@@ -123,18 +199,25 @@ void run_request(Req &r) {
 void worker(ifs_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        // the oldest queued request that fits; the oldest one anyway if nothing runs
+        // among the queued requests whose footprint fits (any one if nothing runs), the one
+        // with the least estimated device time less its waiting time: short requests pass
+        // long ones, and a long one's priority grows as it waits (no starvation)
         int64_t pick = 0;
         s->cv_work.wait(lk, [&] {
             if (s->stop && s->queue.empty()) return true;
+            const clk::time_point now = clk::now();
+            double best = 0.0;
+            pick = 0;
             for (int64_t id : s->queue) {
                 Req &r = s->reqs[id];
-                if (s->admitted + r.footprint <= s->budget || s->running == 0) {
+                if (!(s->admitted + r.footprint <= s->budget || s->running == 0)) continue;
+                const double prio = r.est_s - std::chrono::duration<double>(now - r.t_submit).count();
+                if (!pick || prio < best) {
                     pick = id;
-                    return true;
+                    best = prio;
                 }
             }
-            return false;
+            return pick != 0;
         });
         if (!pick) {
             if (s->stop && s->queue.empty()) return;
@@ -169,6 +252,7 @@ int ifs_server_create(ifs_server **out, int32_t slots, double hbm_budget_gb) {
     if (!out || slots < 1 || slots > 64 || !(hbm_budget_gb > 0.0)) return IFS_E_ARG;
     ifs_server *s = new ifs_server();
     s->budget = (size_t)(hbm_budget_gb * 1073741824.0);
+    s->slots = slots;
     s->t0 = clk::now();
     for (int i = 0; i < slots; ++i) s->workers.emplace_back(worker, s);
     *out = s;
@@ -191,10 +275,14 @@ int ifs_server_submit(ifs_server *s, const ifs_cluster *cl, const ifs_params *pr
     r.prm = *prm;
     r.tol = tol;
     r.restart = restart;
-    size_t tab = 0;
-    r.footprint = estimate_bytes(r, &tab);
-    // a request caches no more tables than its admitted footprint holds
-    if (r.prm.cache_mb == 0) r.prm.cache_mb = (int32_t)std::min<size_t>((size_t)1 << 30, (tab >> 20) + 1);
+    if (r.prm.precision != IFS_PREC_FP64 && r.prm.precision != IFS_PREC_MIXED) return IFS_E_ARG;
+    // the context's workspace and table cache come from the plan, not from free memory
+    {
+        const Plan P = plan_request(r, s->budget, s->slots);
+        r.footprint = P.total();
+        // ~20 matvecs per solve at ~10 TFLOP/s effective, per right-hand side
+        r.est_s = 20.0 * P.flops * (double)r.alpha.size() / 1e13;
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return IFS_E_ARG;
     r.id = s->next_id++;

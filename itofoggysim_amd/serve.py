@@ -16,8 +16,9 @@ class Server:
         self._h = h
 
     def submit(self, centres, radii, m_rel, k, directions, lmax=None, tol=1e-6, restart=30, n_global=None,
-               z_selector=3):
-        """Queues one request; directions = [(alpha, beta), ...]; returns its id."""
+               z_selector=3, use_fmm=None, precision="fp64", leaf_size=32, fmm_d0=6.0):
+        """Queues one request; directions = [(alpha, beta), ...]; returns its id.
+        use_fmm None: the MLFMM above 200 spheres; precision 'fp64' or 'mixed'."""
         centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
         radii = np.ascontiguousarray(radii, dtype=np.float64)
         ns = radii.shape[0]
@@ -31,7 +32,12 @@ class Server:
         pr.k = float(k)
         pr.n_global = int(n_global if n_global is not None else lmax.max())
         pr.z_selector = int(z_selector)
-        pr.use_fmm = 0
+        pr.use_fmm = int(ns > 200 if use_fmm is None else bool(use_fmm))
+        pr.leaf_size = int(leaf_size)
+        pr.fmm_d0 = float(fmm_d0)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        pr.precision = 0 if precision == "fp64" else 1
         d = np.asarray(directions, dtype=np.float64).reshape(-1, 2)
         al = np.ascontiguousarray(d[:, 0])
         be = np.ascontiguousarray(d[:, 1])
@@ -70,3 +76,50 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def water_index(wavelength_um):
+    """n(lambda) = 1.3199 + 3500 nm^2 / lambda^2, k = 1e-8 (lambda in um)."""
+    return complex(1.3199 + 3500e-6 / wavelength_um ** 2, 1e-8)
+
+
+def poisson_workload(n_requests, rate_hz, seed=0, n_min=10, n_max=20000, max_dirs=16, mixed_fraction=0.5):
+    """Requests of the serving workload: arrival times of a Poisson process of the given
+    rate; per request (seed = request id) a cluster of N log-uniform in [n_min, n_max]
+    spheres with the 20-sphere benchmark's radius law (uniform 0.3-1.0 um) and number
+    density (a cube of 10 (N/20)^(1/3) um), lambda uniform in [400, 1550] nm with the water
+    index above, 1..max_dirs incidence directions, tagged plain FP64 or mixed."""
+    from .cluster import make_cluster
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate_hz)
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        lam = float(rng.uniform(0.4, 1.55))
+        m = water_index(lam)
+        side = 10.0 * (n / 20.0) ** (1.0 / 3.0)
+        c = make_cluster(n, ("uniform", 0.3, 1.0), side=side, wavelength=lam, m=m, seed=i)
+        nd = int(rng.integers(1, max_dirs + 1))
+        dirs = [(float(np.arccos(rng.uniform(-1, 1))), float(rng.uniform(0, 2 * np.pi))) for _ in range(nd)]
+        out.append(dict(arrival_s=t, cluster=c, directions=dirs,
+                        precision="mixed" if rng.uniform() < mixed_fraction else "fp64"))
+    return out
+
+
+def run_workload(server, requests, tol=1e-6, restart=30, clock=None):
+    """Submits the requests at their arrival times and waits for all; returns
+    (per-request results, server stats)."""
+    import time
+    clock = clock or time.perf_counter
+    t0 = clock()
+    ids = []
+    for r in requests:
+        delay = r["arrival_s"] - (clock() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        c = r["cluster"]
+        ids.append(server.submit(c["centres"], c["radii"], c["m_rel"], c["k"], r["directions"], lmax=c["lmax"],
+                                 tol=tol, restart=restart, precision=r["precision"]))
+    res = [server.wait(i) for i in ids]
+    return res, server.stats()

@@ -62,3 +62,88 @@ def test_serving_same_alone_and_in_the_mix():
         print(f"ns={len(r['radii'])} ndir={len(r['directions'])} latency {info['latency_s']:.3f} s, "
               f"rel diff vs alone {err:.1e}")
         assert err < 1e-8
+
+
+def _oracle_csec(r, oracle):
+    """Cross sections of a small request from the oracle: the dense operator I + T A
+    assembled from the oracle's pair translations (Gaunt/3j A/B per ordered pair, as its
+    direct matvec applies them, lib_mie_ms_solver_interface.f90:806-998), a direct
+    solve, and the oracle's cross sections."""
+    from oracle import farfield as FF
+    pos, rad, k = r["centres"], r["radii"], r["k"]
+    ns = len(rad)
+    lm = np.ceil(k * rad + 4.05 * np.cbrt(k * rad) + 2.0).astype(np.int32)
+    N = int(lm.max())
+    blk = N * (N + 2)
+    an = np.zeros((ns, N), complex)
+    bn = np.zeros_like(an)
+    for j in range(ns):
+        an[j], bn[j] = oracle.mie_ab(k * rad[j], r["m_rel"], N)
+    nn = np.array([q for q in range(1, N + 1) for _ in range(-q, q + 1)])
+    n = 2 * ns * blk
+    M = np.eye(n, dtype=complex)
+    for j in range(ns):
+        rt = j * 2 * blk
+        nt = lm[j] * (lm[j] + 2)
+        ta, tb = an[j][nn[:nt] - 1], bn[j][nn[:nt] - 1]
+        for l in range(ns):
+            if l == j:
+                continue
+            d = pos[j] - pos[l]  # physical sign: d = origin(target) - origin(source)
+            dd = np.linalg.norm(d)
+            A, B = oracle.translation_ab(k * dd, np.arccos(d[2] / dd), np.arctan2(d[1], d[0]), lm[j], lm[l], 3)
+            cs = l * 2 * blk
+            nc = lm[l] * (lm[l] + 2)
+            M[rt:rt + nt, cs:cs + nc] = ta[:, None] * A
+            M[rt:rt + nt, cs + blk:cs + blk + nc] = ta[:, None] * B
+            M[rt + blk:rt + blk + nt, cs:cs + nc] = tb[:, None] * B
+            M[rt + blk:rt + blk + nt, cs + blk:cs + blk + nc] = tb[:, None] * A
+    out = []
+    for a_, b_ in r["directions"]:
+        p = FF.plane_wave(pos, N, k, a_, b_)
+        rhs = p.copy()
+        for j in range(ns):
+            keep = nn <= lm[j]
+            rhs[2 * j * blk:2 * j * blk + blk] *= np.where(keep, an[j][nn - 1], 0)
+            rhs[2 * j * blk + blk:2 * (j + 1) * blk] *= np.where(keep, bn[j][nn - 1], 0)
+        x = np.linalg.solve(M, rhs)
+        out.append(FF.cross_sections(x, p, an, bn, N, k))
+    return np.array(out)
+
+
+def test_served_cross_sections_vs_oracle(oracle):
+    """Two of the small requests, served (GMRES to 1e-12), against the oracle's dense solve:
+    the cross sections agree to the solution accuracy the tolerance and the operator's
+    conditioning allow (1e-8 of C_ext)."""
+    reqs = _requests()[:2]
+    srv = Server(slots=2, hbm_budget_gb=16.0)
+    ids = [srv.submit(r["centres"], r["radii"], r["m_rel"], r["k"], r["directions"], tol=1e-12) for r in reqs]
+    got = [srv.wait(i) for i in ids]
+    srv.close()
+    for r, (cs, info) in zip(reqs, got):
+        ref = _oracle_csec(r, oracle)
+        # C_sca, C_ext relative; C_abs (k = 1e-8: ~1e-8 of C_ext, a difference of nearly
+        # equal sums) relative to C_ext
+        err = np.max(np.abs(cs - ref) / np.abs(ref[:, 1:2]))
+        print(f"ns={len(r['radii'])} ndir={len(r['directions'])}: served vs oracle dense solve {err:.1e} "
+              f"(per component {np.max(np.abs(cs - ref) / np.abs(ref), axis=0)})")
+        assert info["status"] == 0 and err < 1e-8
+
+
+def test_poisson_workload_mixed_tags():
+    """Poisson arrivals of requests with N log-uniform in [10, 300] (MLFMM above 200),
+    1-16 directions, plain-FP64 and mixed tags: all complete, the footprints admitted at
+    once stay inside the budget whenever more than one request runs, p50 <= p99."""
+    from itofoggysim_amd import poisson_workload, run_workload
+    reqs = poisson_workload(10, 20.0, seed=3, n_min=10, n_max=300)
+    srv = Server(slots=3, hbm_budget_gb=48.0)
+    res, st = run_workload(srv, reqs, tol=1e-6)
+    srv.close()
+    print("poisson workload stats", st)
+    for r, (cs, info) in zip(reqs, res):
+        print(f"N={len(r['cluster']['radii'])} ndir={len(r['directions'])} {r['precision']}: "
+              f"latency {info['latency_s']:.2f} s (queue {info['queue_s']:.2f} s), footprint "
+              f"{info['footprint_bytes'] / 2**30:.2f} GiB")
+        assert info["status"] == 0 and np.all(np.isfinite(cs)) and np.all(cs[:, 1] > 0)
+    assert st["completed"] == len(reqs) and 0 < st["p50_s"] <= st["p99_s"]
+    assert st["peak_concurrent"] == 1 or st["peak_admitted_bytes"] <= 48 * 2 ** 30
