@@ -5,6 +5,9 @@ radii, median 1 um, sigma_g 1.5, clipped to [0.2, 3] um, volume fraction 1e-3,
 lambda 550 nm, seed 2000) through the MLFMM (leaf <= 32 spheres, box truncation
 L = kD + 1.8 d0^(2/3) (kD)^(1/3), d0 = 6), FP64, inputs resident in HBM.
 
+``--config 3 [--wavelength 1.55]``: the same measurement on the 20000-sphere cluster
+(radii uniform in [0.3, 1] um in a 100 um cube, 550 or 1550 nm) with GMRES(30).
+
 Prints one JSON line (rank 0).  The CPU baseline times the oracle (the
 reference's algorithm: full Gaunt/3j translation matrices per sphere pair,
 regenerated every matvec) on a bounded sample of sphere pairs of the same
@@ -60,9 +63,28 @@ def cpu_baseline(cluster, npairs, threads, affinity):
     return {"value": pairs_per_s / (ns * (ns - 1)), "unit": "matvecs/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": affinity,
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "sample": f"{npairs} random ordered sphere pairs of the same 2000-sphere cluster through the oracle "
+            "sample": f"{npairs} random ordered sphere pairs of the same {ns}-sphere cluster through the oracle "
                       f"(full Xu A/B by the Gaunt/3j formula per pair, dense apply) in {dt:.1f} s on {threads} "
                       f"threads, extrapolated to one direct matvec over all {ns * (ns - 1)} ordered pairs"}
+
+
+def solution_errors(s, x64, x32):
+    """Mixed-precision solution against the FP64 one: coefficients (normalised basis),
+    C_sca/C_ext/C_abs of the +z x-polarised plane wave, far field on the 181 x 36
+    (theta, phi) grid (lib_mie_ss_helper_functions.f90:1380-1475)."""
+    import math
+    S = snm_flat(s.n_global, s.n_spheres)
+    p = s.plane_wave(0.0, 0.0)
+    cs64 = np.array(s.cross_sections(x64, p))
+    cs32 = np.array(s.cross_sections(x32, p))
+    th = np.repeat(np.linspace(0.0, math.pi, 181), 36)
+    ph = np.tile(np.linspace(0.0, 2 * math.pi, 36, endpoint=False), 181)
+    f64 = np.concatenate(s.far_field(x64, th, ph))
+    f32 = np.concatenate(s.far_field(x32, th, ph))
+    return {"coefficients": float(np.linalg.norm((x32 - x64) * S) / np.linalg.norm(x64 * S)),
+            "c_sca_ext_abs": [float(v) for v in np.abs(cs32 - cs64) / np.abs(cs64)],
+            "far_field_181x36": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64))),
+            "c_sca_ext_abs_fp64_um2": [float(v) for v in cs64]}
 
 
 def solve_latency(seeds, x0, max_iter, log, n=None, precision="fp64"):
@@ -112,11 +134,15 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES(50) solve to 1e-6")
-    ap.add_argument("--solve-max-iter", type=int, default=100)
+    ap.add_argument("--solve-max-iter", type=int, default=None, help="default 100 (config 2), 400 (config 3)")
     ap.add_argument("--x0", choices=("single", "zero"), default="single",
                     help="GMRES initial guess: the single-sphere solution T_j p_j, i.e. the right-hand side "
                          "(lib_mie_multi_sphere.f90:134-138), or zero")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
+    ap.add_argument("--config", type=int, choices=(2, 3), default=2,
+                    help="2: the 2000-sphere lognormal cluster (default); 3: 20000 spheres U[0.3,1] um at the "
+                         "20-sphere cluster's number density (100 um cube), GMRES(30)")
+    ap.add_argument("--wavelength", type=float, default=0.55, help="config 3 only: 0.55 or 1.55 um")
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="serving workload: Poisson arrivals of this many requests (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="serving arrival rate, requests/s")
@@ -145,7 +171,22 @@ def main():
         return
     from itofoggysim_amd import MultiSphereSolver, config_cluster
 
-    c = config_cluster(2, n=args.n if args.n != 2000 else None)
+    if args.config == 3:
+        c = config_cluster(3, wavelength=args.wavelength)
+        args.latency_seeds = 0
+        args.serve_requests = 0
+        restart = 30
+        args.solve_max_iter = args.solve_max_iter or 400
+        workload = (f"MLFMM matvec of the 20000-sphere water cluster (r uniform in [0.3,1] um, 100 um cube, "
+                    f"lambda {args.wavelength * 1000:.0f} nm, m = {c['m_rel'][0]:.6g}), leaf <= 32, box L = kD + "
+                    f"1.8 d0^(2/3) (kD)^(1/3) with d0 = 6, FP64")
+    else:
+        c = config_cluster(2, n=args.n if args.n != 2000 else None)
+        restart = 50
+        args.solve_max_iter = args.solve_max_iter or 100
+        workload = ("MLFMM matvec of the 2000-sphere water cluster (lognormal r, median 1 um, sigma_g 1.5, "
+                    "[0.2,3] um, volume fraction 1e-3, lambda 550 nm), leaf <= 32, box L = kD + 1.8 d0^(2/3)"
+                    " (kD)^(1/3) with d0 = 6, FP64")
     t0 = time.perf_counter()
     s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"], use_fmm=True,
                           leaf_size=32, fmm_d0=6.0)
@@ -222,7 +263,7 @@ def main():
     # counter runs of this same workload, calibrated on a plain copy (profiles/)
     traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "profiles", "mlfmm_2000_spheres_traffic.json")
-    if os.path.exists(tfile) and args.n == 2000:
+    if os.path.exists(tfile) and args.n == 2000 and args.config == 2:
         tj = json.load(open(tfile))
         t = tj["bytes_per_dispatch"].get(dom)
         if t:
@@ -253,9 +294,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded cluster generator, seed 2000; random seeded x)",
         "config": {
-            "workload": "MLFMM matvec of the 2000-sphere water cluster (lognormal r, median 1 um, sigma_g 1.5, "
-                        "[0.2,3] um, volume fraction 1e-3, lambda 550 nm), leaf <= 32, box L = kD + 1.8 d0^(2/3)"
-                        " (kD)^(1/3) with d0 = 6, FP64",
+            "workload": workload,
             "n_spheres": s.n_spheres, "unknowns_flat": s.n, "box_L_per_level": levels, "translations": counts,
             "matvec_algorithmic_flop": flops_mv, "build_s": round(build_s, 3)},
         "roofline": roofline,
@@ -277,8 +316,10 @@ def main():
         xs = rhs.clone() if args.x0 == "single" else torch.zeros_like(rhs)
         torch.cuda.synchronize()
         with torch.cuda.stream(st):
-            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=args.solve_max_iter, tol=1e-6,
-                                  use_x0=args.x0 == "single", verbose=True)
+            info = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=restart, max_iter=args.solve_max_iter,
+                                  tol=1e-6, use_x0=args.x0 == "single", verbose=True)
+        torch.cuda.synchronize()
+        x64 = xs.cpu().numpy()
         info["initial_guess"] = "single-sphere solution" if args.x0 == "single" else "zero"
         info["note"] = "tables generated in the warm-up matvec (steady-state solve)"
         out["solve"] = info
@@ -286,9 +327,11 @@ def main():
         xs = rhs.clone() if args.x0 == "single" else torch.zeros_like(rhs)
         torch.cuda.synchronize()
         with torch.cuda.stream(st):
-            info32 = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=50, max_iter=args.solve_max_iter,
+            info32 = s.solve_device(rhs.data_ptr(), xs.data_ptr(), restart=restart, max_iter=args.solve_max_iter,
                                     tol=1e-6, use_x0=args.x0 == "single", verbose=True, precision="mixed")
+        torch.cuda.synchronize()
         out["solve_mixed"] = info32
+        out["solve_mixed"]["vs_fp64"] = solution_errors(s, x64, xs.cpu().numpy())
         out["solves_per_s_steady_mixed"] = 1.0 / info32["seconds"] if info32["converged"] else None
         del rhs, xs
     s.close()

@@ -68,15 +68,25 @@ def make_cluster(n, radius_dist, side=None, volume_fraction=None, wavelength=0.5
                 k=2.0 * math.pi * n_medium / wavelength, wavelength=wavelength, side=side, seed=seed)
 
 
-def config_cluster(config, replica=0, n=None):
+WATER_1550 = complex(1.318, 1e-4)
+
+
+def config_cluster(config, replica=0, n=None, wavelength=None):
     """The benchmark clusters: 1 = 20 spheres U[0.3,1] um in a 10 um cube;
-    2 = 2000 spheres lognormal(1 um, 1.5) clipped to [0.2, 3] um at volume fraction 1e-3.
-    ``n`` overrides the sphere count (same generator)."""
+    2 = 2000 spheres lognormal(1 um, 1.5) clipped to [0.2, 3] um at volume fraction 1e-3;
+    3 = the generator of 1 scaled to 20000 spheres at the same number density (a cube of
+    10 (N/20)^(1/3) um, 100 um for 20000), at 550 nm (m = 1.333 + 1e-9 i) or at
+    ``wavelength=1.55`` (m = 1.318 + 1e-4 i).  ``n`` overrides the sphere count (same
+    generator)."""
     seed = 1000 * config + replica
     if config == 1:
         return make_cluster(n or 20, ("uniform", 0.3, 1.0), side=10.0, seed=seed)
     if config == 2:
         return make_cluster(n or 2000, ("lognormal", 1.0, 1.5, 0.2, 3.0), volume_fraction=1e-3, seed=seed)
     if config == 3:
-        return make_cluster(n or 20000, ("uniform", 0.3, 1.0), volume_fraction=None or 1e-3, seed=seed)
+        n = n or 20000
+        lam = wavelength or 0.55
+        m = WATER_1550 if abs(lam - 1.55) < 1e-9 else WATER_550
+        return make_cluster(n, ("uniform", 0.3, 1.0), side=10.0 * (n / 20.0) ** (1.0 / 3.0), wavelength=lam, m=m,
+                            seed=seed)
     raise ValueError(config)

@@ -266,8 +266,8 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
         // one right-hand side and tables not kept: the coaxial coefficients are applied as
         // they are generated and never stored (the table round trip through HBM is what
         // the separate generate/apply pair spends its time on)
-        const bool fused =
-            !B.gq && nrhs == 1 && !B.cached && ch.max_Lt < (ch.max_nmax + 2 <= 128 ? 256 : 1024);
+        const bool fused = !B.gq && nrhs == 1 && !B.cached &&
+                           ch.max_Lt < (ch.max_nmax + 2 <= 128 ? 256 : 1024) && ch.max_nmax <= 380;  // LDS <= 64 KiB
         if (!(B.cached && B.tables_ready)) {
             {
                 Scope s(c, B.pn[0]);
@@ -300,7 +300,7 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
             }
             if (fused) {
                 Scope s(c, B.pn[4], ch.f_ax);
-                launch_axial_fused(tt, nt, ch.maxK, ch.max_nmax, wsc, wy, wz, c->stream);
+                launch_axial_fused(tt, nt, ch.maxK, ch.max_nmax, ch.max_Lt, wsc, wy, wz, c->stream);
             } else {
                 Scope s(c, B.pn[4], ch.f_ax);
                 launch_axial_apply(tt, nt, ch.max_rows_z, wax, wy, wz, c->stream);

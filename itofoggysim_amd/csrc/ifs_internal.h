@@ -182,7 +182,7 @@ void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const d
                         double2 *wz, hipStream_t s);
 // Coaxial shift Y -> Z with the coefficients generated on the fly (needs the
 // sectorial columns of launch_gen_sect; target orders < 1024 when Ls <= Lt).
-void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect,
+void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, int max_Lt, const double2 *wsect,
                         const double2 *wy, double2 *wz, hipStream_t s);
 void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblocks, const double *wd,
                      const double2 *wz, double2 *dst_pool, hipStream_t s);

@@ -371,7 +371,13 @@ __global__ void k_gen_axial(const Task *__restrict__ tasks, int max_nmax, const 
 // so the tables never reach HBM.  Block per (order m >= 0, task) of 64 or 256
 // threads; rows of a thread: FR at most (target orders up to FR * threads - 1).
 // ------------------------------------------------------------------------
-constexpr int FR = 4;
+// Every factor that depends on one order only (a^m_r, 1/a^m_r, 1/sqrt(nu(nu+1)) and the
+// four target-order factors of the vector relation) is tabulated in LDS once per block,
+// and the source rows y(nu, +-m) are staged there too: the serial recurrence then runs
+// on LDS operands and multiply-adds, with no square root, division or global load in
+// its steps.  LDS: 168 (max_nmax + 2) + 256 bytes.  FR: target rows per thread (1 when
+// every target order fits one row per thread, which halves the registers).
+template <int FR>
 __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, const double2 *__restrict__ wsect,
                               const double2 *__restrict__ wy, double2 *__restrict__ wz) {
     extern __shared__ double2 sh[];
@@ -381,32 +387,64 @@ __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, cons
     const int nmax = T.Lt + T.Ls + 2;
     const int Lt = T.Lt, Ls = T.Ls, K = T.K;
     const double t = T.t;
+    const int M2 = max_nmax + 2;
     double2 *P = sh;
-    double2 *C = sh + (max_nmax + 2);
-    double2 *N = sh + 2 * (max_nmax + 2);
-    double2 *red = sh + 3 * (max_nmax + 2);  // 4 waves x 4 partial sums
+    double2 *C = sh + M2;
+    double2 *N = sh + 2 * M2;
+    double2 *red = sh + 3 * M2;  // 4 waves x 4 partial sums
+    double2 *Y0 = red + 16, *Y1 = Y0 + M2, *Y2 = Y1 + M2, *Y3 = Y2 + M2;  // y_+(nu,m) y_-(nu,m) y_+(nu,-m) y_-(nu,-m)
+    double *KA = (double *)(Y3 + M2);  // a^m_r
+    double *IKA = KA + M2;             // 1 / a^m_r
+    double *RN = IKA + M2;             // 1 / sqrt(nu(nu+1))
+    double *K0 = RN + M2;              // sqrt(n(n+1))
+    double *KP = K0 + M2;              // t a^m_n sqrt(n/(n+1))
+    double *KM = KP + M2;              // t a^m_{n-1} sqrt((n+1)/n)
+    double *KB = KM + M2;              // t m / sqrt(n(n+1))
     const double2 *src = wsect + T.off_s + (int64_t)m * (nmax + 2);
-    int top = nmax - m;
-    for (int n = threadIdx.x; n <= nmax + 1; n += blockDim.x) {
-        P[n] = make_double2(0.0, 0.0);
-        C[n] = (n >= m && n <= top) ? src[n] : make_double2(0.0, 0.0);
-        N[n] = make_double2(0.0, 0.0);
-    }
-    __syncthreads();
     const int q = m < 1 ? 1 : m;
     const int64_t ny = rows_size(Ls, K), nz = rows_size(Lt, K);
     const double2 *yp = wy + T.off_y;  // y_+ rows (nu, k); y_- at + ny
     double2 *zp = wz + T.off_z;        // z_+ rows (n, k); z_- at + nz
     auto yrow = [&](int nu, int k) { return rows_off(nu, K) + k + (nu < K ? nu : K); };
+    int top = nmax - m;
+    const double2 zero = make_double2(0.0, 0.0);
+    for (int n = threadIdx.x; n <= nmax + 1; n += blockDim.x) {
+        P[n] = zero;
+        C[n] = (n >= m && n <= top) ? src[n] : zero;
+        N[n] = zero;
+        const double a = coef_a(n, m);
+        KA[n] = a;
+        IKA[n] = a != 0.0 ? 1.0 / a : 0.0;
+        if (n >= q && n <= Ls) {
+            RN[n] = 1.0 / sqrt((double)n * (n + 1));
+            const int64_t iy = yrow(n, m);
+            Y0[n] = yp[iy];
+            Y1[n] = yp[iy + ny];
+            if (m > 0) {
+                const int64_t iyn = yrow(n, -m);
+                Y2[n] = yp[iyn];
+                Y3[n] = yp[iyn + ny];
+            } else {
+                Y2[n] = zero;
+                Y3[n] = zero;
+            }
+        }
+        if (n >= q && n <= Lt) {
+            const double nn = (double)n;
+            K0[n] = sqrt(nn * (nn + 1.0));
+            KP[n] = t * a * sqrt(nn / (nn + 1.0));
+            KM[n] = t * coef_a(n - 1, m) * sqrt((nn + 1.0) / nn);
+            KB[n] = t * (double)m / sqrt(nn * (nn + 1.0));
+        }
+    }
+    __syncthreads();
     // P = A + B, Q = A - B of target order n, source order nu
-    auto coef = [&](int n, int nu, double2 c0, double2 cp, double2 cm, double2 &pv, double2 &qv) {
-        const double nn = (double)n;
-        const double inv_nu = 1.0 / sqrt((double)nu * (nu + 1));
-        double2 s = cscale(sqrt(nn * (nn + 1.0)), c0);
-        s = cadd(s, cscale(t * coef_a(n, m) * sqrt(nn / (nn + 1.0)), cp));
-        s = cadd(s, cscale(t * coef_a(n - 1, m) * sqrt((nn + 1.0) / nn), cm));
+    auto coef = [&](int n, double inv_nu, int nu, double2 c0, double2 cp, double2 cm, double2 &pv, double2 &qv) {
+        double2 s = cscale(K0[n], c0);
+        s = cadd(s, cscale(KP[n], cp));
+        s = cadd(s, cscale(KM[n], cm));
         double2 av = ipow_mul((nu - n) & 3, cscale(inv_nu, s));
-        double2 bv = ipow_mul((nu - n + 1) & 3, cscale(t * (double)m * inv_nu / sqrt(nn * (nn + 1.0)), c0));
+        double2 bv = ipow_mul((nu - n + 1) & 3, cscale(KB[n] * inv_nu, c0));
         pv = cadd(av, bv);
         qv = csub(av, bv);
     };
@@ -416,36 +454,31 @@ __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, cons
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         for (int c = m;; ++c) {
             if (c <= Lt) {
-                const double ac = coef_a(c, m);
-                const double acm1 = coef_a(c - 1, m);
+                const double iac = IKA[c];
+                const double acm1 = c >= 1 ? KA[c - 1] : 0.0;
                 for (int r = m + threadIdx.x; r <= top - 1; r += blockDim.x) {
                     double2 v = cscale(acm1, P[r]);
-                    v = csub(v, cscale(coef_a(r, m), C[r + 1]));
-                    if (r - 1 >= m) v = cadd(v, cscale(coef_a(r - 1, m), C[r - 1]));
-                    N[r] = cscale(1.0 / ac, v);
+                    v = csub(v, cscale(KA[r], C[r + 1]));
+                    if (r - 1 >= m) v = cadd(v, cscale(KA[r - 1], C[r - 1]));
+                    N[r] = cscale(iac, v);
                 }
-                for (int r = top + threadIdx.x; r <= nmax + 1; r += blockDim.x) N[r] = make_double2(0.0, 0.0);
+                for (int r = top + threadIdx.x; r <= nmax + 1; r += blockDim.x) N[r] = zero;
             }
             __syncthreads();
             if (c >= q && c <= Lt) {
                 const int n = c;
-                double2 s4[4] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0),
-                                 make_double2(0.0, 0.0)};
+                double2 s4[4] = {zero, zero, zero, zero};
                 for (int nu = q + threadIdx.x; nu <= Ls; nu += blockDim.x) {
                     const double sg = ((n + nu) & 1) ? -1.0 : 1.0;
                     double2 c0 = cscale(sg, C[nu]);
                     double2 cp = cscale(-sg, N[nu]);
-                    double2 cm = (n - 1 >= m) ? cscale(-sg, P[nu]) : make_double2(0.0, 0.0);
+                    double2 cm = (n - 1 >= m) ? cscale(-sg, P[nu]) : zero;
                     double2 pv, qv;
-                    coef(n, nu, c0, cp, cm, pv, qv);
-                    const int64_t iy = yrow(nu, m);
-                    s4[0] = cadd(s4[0], cmul(pv, yp[iy]));
-                    s4[1] = cadd(s4[1], cmul(qv, yp[iy + ny]));
-                    if (m > 0) {
-                        const int64_t iyn = yrow(nu, -m);
-                        s4[2] = cadd(s4[2], cmul(qv, yp[iyn]));
-                        s4[3] = cadd(s4[3], cmul(pv, yp[iyn + ny]));
-                    }
+                    coef(n, RN[nu], nu, c0, cp, cm, pv, qv);
+                    s4[0] = cadd(s4[0], cmul(pv, Y0[nu]));
+                    s4[1] = cadd(s4[1], cmul(qv, Y1[nu]));
+                    s4[2] = cadd(s4[2], cmul(qv, Y2[nu]));
+                    s4[3] = cadd(s4[3], cmul(pv, Y3[nu]));
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -489,26 +522,20 @@ __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, cons
 #pragma unroll
     for (int j = 0; j < FR; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[j][i] = make_double2(0.0, 0.0);
+        for (int i = 0; i < 4; ++i) acc[j][i] = zero;
     for (int nu = m; nu <= Ls; ++nu) {
         if (nu >= 1) {
-            const int64_t iy = yrow(nu, m);
-            const double2 ypp = yp[iy], ypm = yp[iy + ny];
-            double2 ynp = make_double2(0.0, 0.0), ynm = ynp;
-            if (m > 0) {
-                const int64_t iyn = yrow(nu, -m);
-                ynp = yp[iyn];
-                ynm = yp[iyn + ny];
-            }
+            const double inv_nu = RN[nu];
+            const double2 ypp = Y0[nu], ypm = Y1[nu], ynp = Y2[nu], ynm = Y3[nu];
 #pragma unroll
             for (int j = 0; j < FR; ++j) {
                 const int n = q + threadIdx.x + j * blockDim.x;
                 if (n > Lt) break;
                 double2 c0 = C[n];
                 double2 cp = C[n + 1];
-                double2 cm = (n - 1 >= m) ? C[n - 1] : make_double2(0.0, 0.0);
+                double2 cm = (n - 1 >= m) ? C[n - 1] : zero;
                 double2 pv, qv;
-                coef(n, nu, c0, cp, cm, pv, qv);
+                coef(n, inv_nu, nu, c0, cp, cm, pv, qv);
                 acc[j][0] = cadd(acc[j][0], cmul(pv, ypp));
                 acc[j][1] = cadd(acc[j][1], cmul(qv, ypm));
                 acc[j][2] = cadd(acc[j][2], cmul(qv, ynp));
@@ -516,13 +543,13 @@ __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, cons
             }
         }
         if (nu == Ls) break;
-        const double anu = coef_a(nu, m);
-        const double anum1 = coef_a(nu - 1, m);
+        const double ianu = IKA[nu];
+        const double anum1 = nu >= 1 ? KA[nu - 1] : 0.0;
         for (int n = m + threadIdx.x; n <= top - 1; n += blockDim.x) {
             double2 v = cscale(anum1, P[n]);
-            v = csub(v, cscale(coef_a(n, m), C[n + 1]));
-            if (n - 1 >= m) v = cadd(v, cscale(coef_a(n - 1, m), C[n - 1]));
-            N[n] = cscale(1.0 / anu, v);
+            v = csub(v, cscale(KA[n], C[n + 1]));
+            if (n - 1 >= m) v = cadd(v, cscale(KA[n - 1], C[n - 1]));
+            N[n] = cscale(ianu, v);
         }
         __syncthreads();
         double2 *tmp = P;
@@ -530,7 +557,7 @@ __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, cons
         C = N;
         N = tmp;
         top -= 1;
-        for (int n = top + 1 + threadIdx.x; n <= nmax + 1; n += blockDim.x) C[n] = make_double2(0.0, 0.0);
+        for (int n = top + 1 + threadIdx.x; n <= nmax + 1; n += blockDim.x) C[n] = zero;
         __syncthreads();
     }
 #pragma unroll
@@ -549,13 +576,18 @@ __global__ void k_axial_fused(const Task *__restrict__ tasks, int max_nmax, cons
     }
 }
 
-void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect,
+void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, int max_Lt, const double2 *wsect,
                         const double2 *wy, double2 *wz, hipStream_t s) {
     if (ntasks <= 0) return;
-    size_t shm = sizeof(double2) * (3 * (size_t)(max_nmax + 2) + 16);
+    size_t shm = 168 * (size_t)(max_nmax + 2) + 256;
     // sphere-sphere orders (<= ~60 each) fill one wave; box orders take four
     const int threads = max_nmax + 2 <= 128 ? 64 : 256;
-    hipLaunchKernelGGL(k_axial_fused, dim3(maxK + 1, ntasks), dim3(threads), shm, s, tasks, max_nmax, wsect, wy, wz);
+    if (max_Lt <= threads)
+        hipLaunchKernelGGL(k_axial_fused<1>, dim3(maxK + 1, ntasks), dim3(threads), shm, s, tasks, max_nmax, wsect,
+                           wy, wz);
+    else
+        hipLaunchKernelGGL(k_axial_fused<4>, dim3(maxK + 1, ntasks), dim3(threads), shm, s, tasks, max_nmax, wsect,
+                           wy, wz);
 }
 
 void launch_gen_axial(const Task *tasks, int ntasks, int maxK, int max_nmax, const double2 *wsect, double2 *wax,

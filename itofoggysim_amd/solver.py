@@ -143,13 +143,15 @@ class MultiSphereSolver:
         check(lib().ifs_get_mie(self._h, an.ctypes.data, bn.ctypes.data), "ifs_get_mie")
         return an, bn
 
-    def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True, precision="fp64"):
+    def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True, precision="fp64",
+              verbose=False):
         """Restarted GMRES on the device; returns (x, info dict) (lib_mie_ms_solver_run).
         precision 'mixed': FP32 box translations in the Arnoldi products, FP64 true residual
-        per cycle (iterative refinement)."""
+        per cycle (iterative refinement).  verbose: progress lines on stderr."""
         b = self.get_vector_b() if rhs is None else np.ascontiguousarray(rhs, dtype=np.complex128)
         x = np.zeros(self.n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
-        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None), 0, _prec(precision))
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None), int(bool(verbose)),
+                            _prec(precision))
         inf = _lib.SolveInfo()
         rc = lib().ifs_solve(self._h, b.ctypes.data, x.ctypes.data, self.n, ctypes.byref(op), ctypes.byref(inf))
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
@@ -166,12 +168,13 @@ class MultiSphereSolver:
         check(lib().ifs_matvec_batch(self._h, X.shape[0], X.ctypes.data, B.ctypes.data, self.n), "ifs_matvec_batch")
         return B
 
-    def solve_batch(self, RHS, restart=50, max_iter=1000, tol=1e-6, raise_on_noconv=True, precision="fp64"):
+    def solve_batch(self, RHS, restart=50, max_iter=1000, tol=1e-6, raise_on_noconv=True, precision="fp64",
+                    verbose=False):
         """Lock-step GMRES for the rows of RHS [nrhs, n]; returns (X, [info dicts])."""
         Bm = np.ascontiguousarray(np.atleast_2d(RHS), dtype=np.complex128)
         nr = Bm.shape[0]
         X = np.zeros_like(Bm)
-        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), 0, 0, _prec(precision))
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), 0, int(bool(verbose)), _prec(precision))
         infos = (_lib.SolveInfo * nr)()
         rc = lib().ifs_solve_batch(self._h, nr, Bm.ctypes.data, X.ctypes.data, self.n, ctypes.byref(op), infos)
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
