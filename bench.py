@@ -122,6 +122,120 @@ def solve_latency(seeds, x0, max_iter, log, n=None, precision="fp64"):
             "runs": runs}
 
 
+def fibonacci_directions(n):
+    """n propagation directions (alpha, beta) on a Fibonacci sphere."""
+    i = np.arange(n) + 0.5
+    return np.arccos(1.0 - 2.0 * i / n), np.mod(i * np.pi * (3.0 - np.sqrt(5.0)), 2.0 * np.pi)
+
+
+def config4(args):
+    """Batched right-hand sides and many clusters per launch:
+    (a) a 5000-sphere cluster (radii U[0.3,1] um, 20-sphere number density, 550 nm, MLFMM
+        leaf <= 32, d0 = 6): matvec time per right-hand side for 1, 8 and 32 columns in
+        one batched product (FP64 and mixed), and the first --c4-dirs of 128 Fibonacci
+        directions x TM/TE solved in lock step (mixed, GMRES(30) to 1e-6);
+    (b) --c4-clusters independent clusters (N uniform in [10, 200], the same generator)
+        in one context: one matvec launch for all (eager and HIP-graph replay), and all
+        solved by per-cluster GMRES(30) to 1e-6 in lock step (graph-captured products)."""
+    import torch
+    from itofoggysim_amd import ClusterBatch, MultiSphereSolver, make_cluster
+    out = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "config 4: batched right-hand sides (5000 spheres) and many clusters per launch"}}
+    c = make_cluster(5000, ("uniform", 0.3, 1.0), side=10.0 * (5000 / 20.0) ** (1.0 / 3.0), seed=4000)
+    t0 = time.perf_counter()
+    s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"], use_fmm=True, leaf_size=32,
+                          fmm_d0=6.0)
+    out["cluster5000"] = {"create_s": round(time.perf_counter() - t0, 3), "unknowns_flat": s.n,
+                          "lmax_range": [int(c["lmax"].min()), int(c["lmax"].max())]}
+    print(f"# 5000 spheres: {s.n} unknowns, created in {out['cluster5000']['create_s']} s", file=sys.stderr,
+          flush=True)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    st = torch.cuda.Stream()
+    s.set_stream(st.cuda_stream)
+    per_rhs = {}
+    for prec in ("fp64", "mixed"):
+        s.set_precision(prec)
+        for nr in (1, 8, 32):
+            X = torch.randn(nr, s.n, dtype=torch.complex128, generator=gen).to("cuda")
+            B = torch.empty_like(X)
+            torch.cuda.synchronize()
+            s.calculate_vector_b_batch_device(X.data_ptr(), B.data_ptr(), nr)  # tables, buffers
+            st.synchronize()
+            t0 = time.perf_counter()
+            s.calculate_vector_b_batch_device(X.data_ptr(), B.data_ptr(), nr)
+            st.synchronize()
+            dt = time.perf_counter() - t0
+            per_rhs[f"{prec}_nrhs{nr}_ms_per_rhs"] = round(1000 * dt / nr, 2)
+            print(f"# 5000 spheres {prec}: {nr} columns {1000 * dt:.0f} ms ({1000 * dt / nr:.1f} ms per rhs)",
+                  file=sys.stderr, flush=True)
+            del X, B
+    s.set_precision("fp64")
+    s.set_stream(0)
+    out["cluster5000"]["matvec"] = per_rhs
+    al, be = fibonacci_directions(128)
+    nd = args.c4_dirs
+    rhs = np.stack([s.get_vector_b_dir(al[i], be[i], te=te) for i in range(nd) for te in (False, True)])
+    t0 = time.perf_counter()
+    X, infos = s.solve_batch(rhs, restart=30, tol=1e-6, max_iter=400, precision="mixed", raise_on_noconv=False,
+                             verbose=True)
+    dt = time.perf_counter() - t0
+    out["cluster5000"]["block_solve"] = {
+        "rhs": 2 * nd, "of": 256, "precision": "mixed", "seconds": round(dt, 3),
+        "solves_per_s": round(2 * nd / dt, 4), "all_converged": all(i["converged"] for i in infos),
+        "iterations": [i["iterations"] for i in infos], "max_rel_residual": max(i["rel_residual"] for i in infos),
+        "extrapolated_256_rhs_s": round(dt * 256 / (2 * nd), 1)}
+    print(f"# 5000 spheres: {2 * nd} right-hand sides solved in lock step in {dt:.1f} s", file=sys.stderr,
+          flush=True)
+    s.close()
+    # (b) many clusters in one context
+    rng = np.random.default_rng(256)
+    sizes = rng.integers(10, 201, args.c4_clusters)
+    t0 = time.perf_counter()
+    cls = [make_cluster(int(n), ("uniform", 0.3, 1.0), side=10.0 * (n / 20.0) ** (1.0 / 3.0), seed=5000 + i)
+           for i, n in enumerate(sizes)]
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mb = ClusterBatch(cls, cls[0]["k"])
+    t_create = time.perf_counter() - t0
+    print(f"# {len(cls)} clusters ({mb.n_spheres} spheres, {mb.n} unknowns): generated in {t_gen:.1f} s, "
+          f"context in {t_create:.1f} s", file=sys.stderr, flush=True)
+    x = torch.randn(mb.n, dtype=torch.complex128, generator=gen).to("cuda")
+    b = torch.empty_like(x)
+    torch.cuda.synchronize()
+    mb.set_stream(st.cuda_stream)
+    times = {}
+    for mode in ("eager", "graph"):
+        f = mb.calculate_vector_b_device if mode == "eager" else mb.calculate_vector_b_graph_device
+        for _ in range(2):
+            f(x.data_ptr(), b.data_ptr())
+        st.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            f(x.data_ptr(), b.data_ptr())
+        st.synchronize()
+        times[mode] = (time.perf_counter() - t0) / 3
+    mb.set_stream(0)
+    rhs = mb.get_vector_b()
+    t0 = time.perf_counter()
+    xs, infos = mb.solve(rhs, restart=30, tol=1e-6, max_iter=400, x0=rhs, graph=True, raise_on_noconv=False,
+                         verbose=True)
+    dt = time.perf_counter() - t0
+    out["clusters"] = {
+        "clusters": len(cls), "spheres": mb.n_spheres, "unknowns_flat": mb.n, "n_range": [int(sizes.min()),
+                                                                                           int(sizes.max())],
+        "generate_s": round(t_gen, 2), "create_s": round(t_create, 2),
+        "matvec_ms": {k: round(1000 * v, 2) for k, v in times.items()},
+        "solve_s": round(dt, 3), "solves_per_s": round(len(cls) / dt, 3),
+        "all_converged": all(i["converged"] for i in infos),
+        "iterations_max": max(i["iterations"] for i in infos),
+        "max_rel_residual": max(i["rel_residual"] for i in infos)}
+    out["value"] = out["clusters"]["solves_per_s"]
+    out["unit"] = "solves/s"
+    print(f"# {len(cls)} clusters solved in {dt:.1f} s", file=sys.stderr, flush=True)
+    mb.close()
+    return out
+
+
 def main():
     # the CPU baseline's OpenMP threads: all cores, bound close (read at libgomp load; the
     # binding pins the main thread, so the CPU set is read first)
@@ -139,9 +253,13 @@ def main():
                     help="GMRES initial guess: the single-sphere solution T_j p_j, i.e. the right-hand side "
                          "(lib_mie_multi_sphere.f90:134-138), or zero")
     ap.add_argument("--n", type=int, default=2000, help="sphere count (default: the 2000-sphere cluster)")
-    ap.add_argument("--config", type=int, choices=(2, 3), default=2,
+    ap.add_argument("--config", type=int, choices=(2, 3, 4), default=2,
                     help="2: the 2000-sphere lognormal cluster (default); 3: 20000 spheres U[0.3,1] um at the "
-                         "20-sphere cluster's number density (100 um cube), GMRES(30)")
+                         "20-sphere cluster's number density (100 um cube), GMRES(30); 4: batched right-hand "
+                         "sides of a 5000-sphere cluster and 256 clusters of N in [10, 200] in one context")
+    ap.add_argument("--c4-dirs", type=int, default=4,
+                    help="config 4: incidence directions (x TM/TE) solved in lock step out of the 128")
+    ap.add_argument("--c4-clusters", type=int, default=256, help="config 4: independent clusters in one context")
     ap.add_argument("--wavelength", type=float, default=0.55, help="config 3 only: 0.55 or 1.55 um")
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="serving workload: Poisson arrivals of this many requests (0 skips it)")
@@ -171,6 +289,9 @@ def main():
         return
     from itofoggysim_amd import MultiSphereSolver, config_cluster
 
+    if args.config == 4:
+        print(json.dumps(config4(args)))
+        return
     if args.config == 3:
         c = config_cluster(3, wavelength=args.wavelength)
         args.latency_seeds = 0

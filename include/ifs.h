@@ -105,6 +105,13 @@ int ifs_matvec_device(ifs_ctx *ctx, const void *x_dev, void *b_dev, int64_t n);
 int ifs_matvec_batch_device(ifs_ctx *ctx, int32_t nrhs, const void *x_dev, void *b_dev, int64_t n, int64_t ld);
 /* Same on host buffers [nrhs][n]. */
 int ifs_matvec_batch(ifs_ctx *ctx, int32_t nrhs, const double *x, double *b, int64_t n);
+/* ifs_matvec_device through a HIP graph of its launches: the first call with a given
+ * (x_dev, b_dev) pair runs eagerly (sizing every buffer and building the kept tables),
+ * the second captures the launches on the context's stream, later calls replay them
+ * (one hipGraphLaunch).  A graph is dropped and captured again when a buffer or plan
+ * it refers to has changed; profiling (ifs_profile_enable) runs eagerly.  No
+ * counterpart in the reference (its matvec is a Fortran callback). */
+int ifs_matvec_graph_device(ifs_ctx *ctx, const void *x_dev, void *b_dev, int64_t n);
 
 /* Arithmetic of the context's ifs_matvec* calls from now on (IFS_PREC_FP64 or
  * IFS_PREC_MIXED; ifs_params.precision at creation).  The tables are shared. */
@@ -132,6 +139,11 @@ int ifs_plane_wave(ifs_ctx *ctx, double alpha, double beta, double *p, int64_t n
 /* rhs = T p for that wave; rows beyond a sphere's L_j (and of inactive spheres) are 0.
  * Generalises ifs_plane_wave_rhs (alpha = beta = 0). */
 int ifs_plane_wave_rhs_dir(ifs_ctx *ctx, double alpha, double beta, double *rhs, int64_t n);
+/* The two polarisations of the reference's illumination: te = 0 is the wave above
+ * (TM), te = 1 the same wave with p and q exchanged (TE, lib_mie_illumination.f90:
+ * 430-470 and 347-352).  Coefficients p, and rhs = T p. */
+int ifs_plane_wave_mode(ifs_ctx *ctx, double alpha, double beta, int32_t te, double *p, int64_t n);
+int ifs_plane_wave_rhs_mode(ifs_ctx *ctx, double alpha, double beta, int32_t te, double *rhs, int64_t n);
 
 /* Cluster cross sections from the solved coefficients x and the incident p of the same
  * wave (flat layout): out = {C_sca, C_ext, C_abs} in the squared length unit of 1/k.
@@ -177,8 +189,12 @@ typedef struct {
                              in mixed precision (ifs_params.precision), the true residual of every
                              restart cycle in FP64, so the cycles refine the iterate in FP64 until the
                              FP64 residual meets tol (iterative refinement) */
-    int32_t reserved;
+    int32_t flags;        /* IFS_SOLVE_GRAPH: the products replay a HIP graph of the matvec (captured
+                             on the context's stream at the second product; one staging vector
+                             carries the Krylov vector into it).  0: eager launches */
 } ifs_solve_opts;
+
+#define IFS_SOLVE_GRAPH 1
 
 typedef struct {
     int32_t iterations;        /* matvecs performed */
@@ -206,6 +222,27 @@ int ifs_solve_device(ifs_ctx *ctx, const void *rhs_dev, void *x_dev, int64_t n, 
 int ifs_solve_batch_device(ifs_ctx *ctx, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n, int64_t ld,
                            const ifs_solve_opts *opts, ifs_solve_info *info);
 int ifs_solve_batch(ifs_ctx *ctx, int32_t nrhs, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
+                    ifs_solve_info *info);
+
+/* Many independent clusters in one context: the spheres of clusters[0..nclusters)
+ * in order, each cluster interacting only with itself (a block-diagonal operator), so
+ * that one matvec launch covers every cluster's pairs.  Direct plans only
+ * (params->use_fmm = 0, IFS_E_ARG otherwise); one wavenumber and one n_global for
+ * all; per-sphere orders (lmax) for every cluster or for none.  The flat vector is the
+ * clusters' flat vectors one after another.  No counterpart in the reference, which
+ * solves one cluster per run (lib_mie_multi_sphere.f90:111-235). */
+int ifs_create_multi(ifs_ctx **out, int32_t nclusters, const ifs_cluster *clusters, const ifs_params *params);
+/* Clusters of a context (1 for ifs_create), and where each starts:
+ * first_sphere[nclusters + 1], first_entry[nclusters + 1] (flat complex entries);
+ * either pointer may be NULL. */
+int32_t ifs_cluster_count(const ifs_ctx *ctx);
+int ifs_cluster_offsets(const ifs_ctx *ctx, int64_t *first_sphere, int64_t *first_entry);
+/* Every cluster of the context solved on its own Krylov space by restarted GMRES in
+ * lock step (one product per Arnoldi step for all clusters; per-cluster
+ * orthogonalisation, convergence and info[nclusters]).  Device and host buffers. */
+int ifs_solve_multi_device(ifs_ctx *ctx, const void *rhs_dev, void *x_dev, int64_t n, const ifs_solve_opts *opts,
+                           ifs_solve_info *info);
+int ifs_solve_multi(ifs_ctx *ctx, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
                     ifs_solve_info *info);
 
 /* Octree export for bit-exact checks (lib_tree.f90:147-270, lib_ml_fmm_helper_functions.f90:366-446).

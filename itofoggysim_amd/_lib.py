@@ -41,7 +41,10 @@ class Params(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("restart", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
                 ("use_x0", ctypes.c_int32), ("verbose", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("flags", ctypes.c_int32)]
+
+
+IFS_SOLVE_GRAPH = 1
 
 
 class RequestInfo(ctypes.Structure):
@@ -104,6 +107,14 @@ SIGNATURES = {
     "ifs_server_stats": (_i32, [_vp, _vp]),
     "ifs_server_destroy": (_i32, [_vp]),
     "ifs_tree_build_gpu": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "ifs_matvec_graph_device": (_i32, [_vp, _vp, _vp, _i64]),
+    "ifs_plane_wave_mode": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _i32, _vp, _i64]),
+    "ifs_plane_wave_rhs_mode": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _i32, _vp, _i64]),
+    "ifs_create_multi": (_i32, [ctypes.POINTER(_vp), _i32, ctypes.POINTER(Cluster), ctypes.POINTER(Params)]),
+    "ifs_cluster_count": (_i32, [_vp]),
+    "ifs_cluster_offsets": (_i32, [_vp, _vp, _vp]),
+    "ifs_solve_multi_device": (_i32, [_vp, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "ifs_solve_multi": (_i32, [_vp, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "ifs_tree_i4_gpu": (_i32, [_vp, _i64, _i32, _vp, _vp]),
 }
 

@@ -823,7 +823,55 @@ int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **
 // ---------------------------------------------------------------------------
 extern "C" {
 
+static int create_impl(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p, const int32_t *group,
+                       int32_t nclusters);
+
 int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
+    return create_impl(out, cl, p, nullptr, 1);
+}
+
+int ifs_create_multi(ifs_ctx **out, int32_t nclusters, const ifs_cluster *cls, const ifs_params *p) {
+    if (!out || !cls || !p || nclusters < 1) return IFS_E_ARG;
+    *out = nullptr;
+    if (p->use_fmm) return IFS_E_ARG;  // independent clusters: direct plans only
+    std::vector<double> centres, radii, mrel;
+    std::vector<int32_t> lmax, group;
+    bool any_l = false, all_l = true;
+    for (int g = 0; g < nclusters; ++g) {
+        const ifs_cluster &q = cls[g];
+        if (q.n_spheres < 1 || !q.centres || !q.radii || !q.m_rel) return IFS_E_ARG;
+        any_l = any_l || q.lmax;
+        all_l = all_l && q.lmax;
+    }
+    if (any_l && !all_l) return IFS_E_ARG;  // per-sphere orders for every cluster or for none
+    for (int g = 0; g < nclusters; ++g) {
+        const ifs_cluster &q = cls[g];
+        centres.insert(centres.end(), q.centres, q.centres + 3 * q.n_spheres);
+        radii.insert(radii.end(), q.radii, q.radii + q.n_spheres);
+        mrel.insert(mrel.end(), q.m_rel, q.m_rel + 2 * q.n_spheres);
+        if (all_l) lmax.insert(lmax.end(), q.lmax, q.lmax + q.n_spheres);
+        group.insert(group.end(), (size_t)q.n_spheres, g);
+    }
+    if (group.size() > (size_t)INT32_MAX) return IFS_E_ARG;
+    ifs_cluster all{(int32_t)group.size(), centres.data(), radii.data(), mrel.data(),
+                    all_l ? lmax.data() : nullptr};
+    return create_impl(out, &all, p, group.data(), nclusters);
+}
+
+int32_t ifs_cluster_count(const ifs_ctx *c) { return c ? (int32_t)c->grp_first.size() - 1 : -1; }
+
+int ifs_cluster_offsets(const ifs_ctx *c, int64_t *first_sphere, int64_t *first_entry) {
+    if (!c) return IFS_E_ARG;
+    const int64_t blk = 2LL * c->N * (c->N + 2);
+    for (size_t g = 0; g < c->grp_first.size(); ++g) {
+        if (first_sphere) first_sphere[g] = c->grp_first[g];
+        if (first_entry) first_entry[g] = c->grp_first[g] * blk;
+    }
+    return IFS_OK;
+}
+
+static int create_impl(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p, const int32_t *group,
+                       int32_t nclusters) {
     if (!out || !cl || !p) return IFS_E_ARG;
     *out = nullptr;
     if (cl->n_spheres < 1 || !cl->centres || !cl->radii || !cl->m_rel) return IFS_E_ARG;
@@ -845,6 +893,16 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
     c->ns = cl->n_spheres;
     c->N = p->n_global;
     const int ns = c->ns, N = c->N;
+    if (group) {
+        c->group.assign(group, group + ns);
+        c->grp_first.assign(1, 0);
+        for (int j = 1; j < ns; ++j)
+            if (group[j] != group[j - 1]) c->grp_first.push_back(j);
+        c->grp_first.push_back(ns);
+        if ((int)c->grp_first.size() != nclusters + 1) return fail(IFS_E_ARG);
+    } else {
+        c->grp_first = {0, ns};
+    }
     c->pos.assign(cl->centres, cl->centres + 3 * ns);
     c->radius.assign(cl->radii, cl->radii + ns);
     c->mrel.resize(ns);
@@ -924,10 +982,17 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
         // direct all-pairs plan: for each target j, sources l != j in ascending order
         Batch &B = c->direct;
         B.name = "direct";
-        B.tasks.reserve((size_t)ns * (ns - 1));
+        size_t npairs = 0;
+        for (size_t g = 0; g + 1 < c->grp_first.size(); ++g) {
+            const size_t q = (size_t)(c->grp_first[g + 1] - c->grp_first[g]);
+            npairs += q * (q - 1);
+        }
+        B.tasks.reserve(npairs);
         for (int j = 0; j < ns; ++j) {
             if (!c->active[j]) continue;
-            for (int l = 0; l < ns; ++l) {
+            // sources: the spheres of the target's own cluster
+            const int lo = group ? c->grp_first[group[j]] : 0, hi = group ? c->grp_first[group[j] + 1] : ns;
+            for (int l = lo; l < hi; ++l) {
                 if (l == j) continue;
                 double dv[3] = {c->dsign * (c->pos[3 * l] - c->pos[3 * j]),
                                 c->dsign * (c->pos[3 * l + 1] - c->pos[3 * j + 1]),
@@ -952,6 +1017,7 @@ int ifs_create(ifs_ctx **out, const ifs_cluster *cl, const ifs_params *p) {
 int ifs_destroy(ifs_ctx *c) {
     if (!c) return IFS_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    graphs_clear(c);
     if (c->fmm) fmm::destroy(c->fmm);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -1013,6 +1079,69 @@ int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nr
     return IFS_OK;
 }
 
+static uint64_t matvec_sig(ifs_ctx *c) {
+    uint64_t h = 0;
+    auto mix = [&](uint64_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
+    for (const DevBuf *d : {&c->d_xh, &c->d_acc, &c->ws_d, &c->ws_s, &c->ws_ax, &c->ws_y, &c->ws_z, &c->ws_gq,
+                            &c->direct.d_tasks, &c->direct.d_segs, &c->direct.own_d, &c->direct.own_s,
+                            &c->direct.own_ax, &c->direct.own_y, &c->direct.own_z, &c->d_snm, &c->d_an, &c->d_bn})
+        mix((uint64_t)(uintptr_t)d->p);
+    for (auto &kv : c->gq_nodes) mix((uint64_t)(uintptr_t)kv.second.p);
+    if (c->fmm) mix(fmm::buffers_sig(c->fmm));
+    return h;
+}
+
+void graphs_clear(ifs_ctx *c) {
+    for (auto &g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+int matvec_graph(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb,
+                 int prec) {
+    if (ldb < 0) ldb = ld;
+    if (c->prof.on || !c->stream) return matvec_impl(c, x, b, normalised, nrhs, ld, ldb, prec);
+    ifs_ctx::Graph *g = nullptr;
+    for (auto &e : c->graphs)
+        if (e.x == x && e.b == b && e.normalised == normalised && e.nrhs == nrhs && e.ld == ld && e.ldb == ldb &&
+            e.prec == prec)
+            g = &e;
+    const uint64_t sig = matvec_sig(c);
+    if (g && g->exec && g->sig == sig)
+        return hipGraphLaunch(g->exec, c->stream) == hipSuccess ? IFS_OK : IFS_E_HIP;
+    if (!g) {
+        // first call with these arguments: eager (sizes every buffer, builds kept tables)
+        int rc = matvec_impl(c, x, b, normalised, nrhs, ld, ldb, prec);
+        if (rc) return rc;
+        c->graphs.push_back({x, b, normalised, nrhs, prec, ld, ldb, 0, nullptr});
+        return IFS_OK;
+    }
+    if (g->exec) {  // a buffer or plan changed since the capture
+        (void)hipGraphExecDestroy(g->exec);
+        g->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return matvec_impl(c, x, b, normalised, nrhs, ld, ldb, prec);
+    }
+    int rc = matvec_impl(c, x, b, normalised, nrhs, ld, ldb, prec);
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    const bool ok = rc == IFS_OK && e == hipSuccess && graph && matvec_sig(c) == sig &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        // nothing ran: the launches went into the capture; run them eagerly
+        (void)hipGetLastError();
+        if (exec) (void)hipGraphExecDestroy(exec);
+        return matvec_impl(c, x, b, normalised, nrhs, ld, ldb, prec);
+    }
+    g->exec = exec;
+    g->sig = sig;
+    return hipGraphLaunch(exec, c->stream) == hipSuccess ? IFS_OK : IFS_E_HIP;
+}
+
 // ||S^-1 r_hat|| / ||S^-1 b_hat|| (the relative residual in the flat basis) from the
 // normalised residual and right-hand side on the device, summed on the host in long
 // double (flat entries reach 1e+-80 at N = 50, their squares overflow doubles at N ~ 80).
@@ -1045,6 +1174,11 @@ extern "C" {
 int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
     if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
     return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n, -1, c->prm.precision);
+}
+
+int ifs_matvec_graph_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
+    if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
+    return matvec_graph(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n, -1, c->prm.precision);
 }
 
 int ifs_matvec_batch_device(ifs_ctx *c, int32_t nrhs, const void *x_dev, void *b_dev, int64_t n, int64_t ld) {

@@ -130,6 +130,8 @@ struct ifs_ctx {
     std::vector<int32_t> Leff;      // min(L_j, N)
     std::vector<int32_t> active;    // row activity (reference row zeroing)
     std::vector<int64_t> off;       // compact normalised offsets
+    std::vector<int32_t> group;     // cluster of each sphere (a multi-cluster context), else empty
+    std::vector<int32_t> grp_first; // first sphere of each cluster, and ns (size clusters + 1)
     int64_t ncompact = 0;           // complex entries of the compact vector
     int64_t nflat = 0;              // complex entries of the flat vector
 
@@ -154,6 +156,18 @@ struct ifs_ctx {
 
     ifs::Profiler prof;
     double flops_mv = 0.0, bytes_mv = 0.0;
+
+    // HIP graphs of the matvec, one per argument set; a graph is replayed while the
+    // signature of the buffers and plans it captured still holds
+    struct Graph {
+        const void *x;
+        void *b;
+        int normalised, nrhs, prec;
+        int64_t ld, ldb;
+        uint64_t sig;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
 };
 
 
@@ -212,6 +226,12 @@ struct ClassBatch {
 // prec: 0 FP64, 1 FP32 class products (M2M, M2L, L2L); everything else stays FP64.
 int matvec_impl(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb = -1,
                 int prec = 0);
+// matvec_impl through a HIP graph: the first call with an argument set runs eagerly
+// (it sizes every buffer and builds the kept tables), the second captures the launches
+// on the context's stream and replays them, later calls replay.
+int matvec_graph(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int nrhs, int64_t ld, int64_t ldb = -1,
+                 int prec = 0);
+void graphs_clear(ifs_ctx *c);
 int class_batch_finalize(ifs_ctx *c, ClassBatch &B);
 // Device Gauss-Legendre rule of Q nodes (cached per context): mu, then w.
 int gq_rule(ifs_ctx *c, int Q, const double **mu, const double **w);

@@ -343,6 +343,25 @@ int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t l
     return IFS_OK;
 }
 
+uint64_t buffers_sig(Fmm *f) {
+    uint64_t h = 0;
+    auto mix = [&](uint64_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
+    auto buf = [&](const DevBuf &d) { mix((uint64_t)(uintptr_t)d.p); };
+    buf(f->C);
+    buf(f->D);
+    for (const Batch *B : {&f->p2m, &f->l2p, &f->p2p})
+        for (const DevBuf *d : {&B->d_tasks, &B->d_segs, &B->own_d, &B->own_s, &B->own_ax, &B->own_y, &B->own_z})
+            buf(*d);
+    for (const auto *v : {&f->m2m, &f->m2l, &f->l2l})
+        for (const ClassBatch &B : *v) {
+            mix((uint64_t)B.plan_nrhs);
+            for (const DevBuf *d : {&B.own_d, &B.own_ax, &B.d_dkeys, &B.d_desc, &B.d_gen, &B.d_src, &B.d_pidx,
+                                    &B.d_phase, &B.d_scat, &B.d_scat_cols})
+                buf(*d);
+        }
+    return h;
+}
+
 int tree_export(Fmm *f, int64_t *uindex_out, int32_t *levels_out) {
     for (size_t i = 0; i < f->leaf_of.size(); ++i) uindex_out[i] = f->leaf_of[i];
     levels_out[0] = f->l_min;

@@ -21,6 +21,8 @@ void destroy(Fmm *f);
 // prec: precision of the class products (see launch_class).
 int run(ifs_ctx *c, Fmm *f, const double2 *xh, double2 *acc, int nrhs, int64_t ld, int prec = 0);
 int tree_export(Fmm *f, int64_t *uindex_out, int32_t *levels_out);
+// Hash of the device buffers and plan shapes a matvec's launches refer to.
+uint64_t buffers_sig(Fmm *f);
 int stats(Fmm *f, int64_t counts[6], int32_t levels_L[32]);
 
 }  // namespace fmm

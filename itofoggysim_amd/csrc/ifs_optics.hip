@@ -323,8 +323,26 @@ extern "C" int ifs_plane_wave(ifs_ctx *c, double alpha, double beta, double *p_o
     return IFS_OK;
 }
 
+// TE mode of the reference's illumination: the TM coefficients with p and q exchanged
+// (lib_mie_illumination.f90:347-352; the polarisation turned by 90 degrees about the
+// propagation direction).
+extern "C" int ifs_plane_wave_mode(ifs_ctx *c, double alpha, double beta, int32_t te, double *p_out, int64_t n) {
+    if (te != 0 && te != 1) return IFS_E_ARG;
+    int rc = ifs_plane_wave(c, alpha, beta, p_out, n);
+    if (rc || !te) return rc;
+    const int64_t blk = (int64_t)c->N * (c->N + 2);
+    cplx *out = (cplx *)p_out;
+    for (int j = 0; j < c->ns; ++j)
+        for (int64_t e = 0; e < blk; ++e) std::swap(out[(int64_t)j * 2 * blk + e], out[(int64_t)j * 2 * blk + blk + e]);
+    return IFS_OK;
+}
+
 extern "C" int ifs_plane_wave_rhs_dir(ifs_ctx *c, double alpha, double beta, double *rhs, int64_t n) {
-    int rc = ifs_plane_wave(c, alpha, beta, rhs, n);
+    return ifs_plane_wave_rhs_mode(c, alpha, beta, 0, rhs, n);
+}
+
+extern "C" int ifs_plane_wave_rhs_mode(ifs_ctx *c, double alpha, double beta, int32_t te, double *rhs, int64_t n) {
+    int rc = ifs_plane_wave_mode(c, alpha, beta, te, rhs, n);
     if (rc) return rc;
     const int N = c->N, ns = c->ns;
     const int64_t blk = (int64_t)N * (N + 2);

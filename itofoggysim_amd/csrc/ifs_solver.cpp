@@ -95,9 +95,15 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         if (info) *info = {0, 1, 0.0, 0.0, 0.0};
         return IFS_OK;
     }
+    // graph mode: the Arnoldi products go through one staging vector, so that one
+    // captured matvec serves every Krylov vector
+    const bool graph = (opts->flags & IFS_SOLVE_GRAPH) != 0;
+    DevBuf dstage;
+    if (graph && (rc = dstage.alloc(16ull * n))) return rc;
+    double2 *stage = dstage.as<double2>();
     // r = b - M x
     auto residual = [&]() -> int {
-        int q = matvec_impl(c, x, r, 1, 1, n);
+        int q = graph ? matvec_graph(c, x, r, 1, 1, n) : matvec_impl(c, x, r, 1, 1, n);
         if (q) return q;
         ++iters;
         launch_axpby(one, b, mone, r, n, s);
@@ -121,7 +127,13 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
         int jend = 0;
         for (int j = 0; j < m; ++j) {
             double2 *vj = V + (int64_t)j * n;
-            if ((rc = matvec_impl(c, vj, w, 1, 1, n, -1, kprec))) return rc;
+            if (graph) {
+                launch_scale_copy(one, vj, stage, n, s);
+                rc = matvec_graph(c, stage, w, 1, 1, n, -1, kprec);
+            } else {
+                rc = matvec_impl(c, vj, w, 1, 1, n, -1, kprec);
+            }
+            if (rc) return rc;
             ++iters;
             // CGS with one re-orthogonalisation pass
             if ((rc = dots(s, V, n, j + 1, w, n, h, part, hv))) return rc;
@@ -212,69 +224,88 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     return breakdown ? IFS_E_BREAKDOWN : IFS_E_NOCONV;
 }
 
-// Restarted GMRES for nrhs right-hand sides in lock step: every Arnoldi step applies
-// the operator to the current Krylov vector of all right-hand sides in one batched
-// matvec (the translation tables serve every column); orthogonalisation (CGS2),
-// Givens rotations and the true-residual test stay per right-hand side.  A right-
-// hand side that converged keeps riding along in the batched matvec until all are
-// done, and its iterate no longer changes.
-extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n,
-                                      int64_t ld, const ifs_solve_opts *opts, ifs_solve_info *info) {
-    if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c) || nrhs < 1 || ld < n) return IFS_E_ARG;
+namespace {
+
+// Independent linear systems advanced in lock step.  The vectors the operator acts on
+// are `ncols` columns of length `ncol`; system r is the entries [off[r], off[r] +
+// len[r]) of column col[r].  Many right-hand sides of one cluster: one system per
+// column, each the whole column.  Independent clusters in one context (a block-
+// diagonal operator): one column, one system per cluster's entries.
+struct Systems {
+    int ncols = 1;
+    int64_t ncol = 0;
+    std::vector<int> col;
+    std::vector<int64_t> off, len;
+    int size() const { return (int)col.size(); }
+};
+
+// Restarted GMRES for every system in lock step: each Arnoldi step applies the
+// operator to the current Krylov vectors of all systems in one matvec call (one
+// batched product over the columns; the translation tables serve every column and
+// every cluster); orthogonalisation (CGS2), Givens rotations and the true-residual
+// test stay per system.  A system that converged keeps riding along in the products
+// until all are done, and its iterate no longer changes.
+int solve_systems(ifs_ctx *c, const Systems &L, const double2 *bin, double2 *xout, int64_t ld,
+                  const ifs_solve_opts *opts, ifs_solve_info *info) {
     const int m = opts->restart;
-    if (m < 1 || opts->max_iter < 1 || !(opts->tol > 0.0)) return IFS_E_ARG;
-    if (opts->precision != IFS_PREC_FP64 && opts->precision != IFS_PREC_MIXED) return IFS_E_ARG;
     const int kprec = opts->precision == IFS_PREC_MIXED ? 1 : 0;
+    const int ns = L.size(), nc = L.ncols;
+    const int64_t n = L.ncol;
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = ctx_stream(c);
     DevBuf dV, dW, dR, dB, dX, dpart, dh;
     int rc;
-    const int64_t ldv = (int64_t)(m + 1) * n;  // V[r][j][n]
-    if ((rc = dV.alloc(16ull * ldv * nrhs)) || (rc = dW.alloc(16ull * n * nrhs)) || (rc = dR.alloc(16ull * n * nrhs)) ||
-        (rc = dB.alloc(16ull * n * nrhs)) || (rc = dX.alloc(16ull * n * nrhs)) ||
+    const int64_t ldv = (int64_t)(m + 1) * n;  // V[column][j][n]
+    if ((rc = dV.alloc(16ull * ldv * nc)) || (rc = dW.alloc(16ull * n * nc)) || (rc = dR.alloc(16ull * n * nc)) ||
+        (rc = dB.alloc(16ull * n * nc)) || (rc = dX.alloc(16ull * n * nc)) ||
         (rc = dpart.alloc(16ull * kDotBlocks * (m + 1))) || (rc = dh.alloc(16ull * (m + 1))))
         return rc;
     double2 *V = dV.as<double2>(), *W = dW.as<double2>(), *R = dR.as<double2>(), *B = dB.as<double2>(),
             *X = dX.as<double2>(), *part = dpart.as<double2>(), *h = dh.as<double2>();
-    launch_zero(V, ldv * nrhs, s);
+    launch_zero(V, ldv * nc, s);
+    const bool graph = (opts->flags & IFS_SOLVE_GRAPH) != 0;
+    DevBuf dstage;
+    if (graph && (rc = dstage.alloc(16ull * n * nc))) return rc;
+    double2 *stage = dstage.as<double2>();
+    // system r's entries in a column vector, and its Krylov vector j
+    auto at = [&](double2 *base, int r) { return base + (int64_t)L.col[r] * n + L.off[r]; };
+    auto kv = [&](int r, int j) { return V + (int64_t)L.col[r] * ldv + (int64_t)j * n + L.off[r]; };
     const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
-    const double2 *bin = (const double2 *)rhs_dev;
-    double2 *xout = (double2 *)x_dev;
-    std::vector<double> bnorm(nrhs), beta(nrhs), flat(nrhs, 0.0);
-    std::vector<int> iters(nrhs, 0), conv(nrhs, 0), brk(nrhs, 0);
-    for (int r = 0; r < nrhs; ++r) {
-        if ((rc = scale_flat(c, bin + r * ld, B + r * n, 0))) return rc;
+    std::vector<double> bnorm(ns), beta(ns), flat(ns, 0.0);
+    std::vector<int> iters(ns, 0), conv(ns, 0), brk(ns, 0);
+    for (int q = 0; q < nc; ++q) {
+        if ((rc = scale_flat(c, bin + q * ld, B + q * n, 0))) return rc;
         if (opts->use_x0) {
-            if ((rc = scale_flat(c, xout + r * ld, X + r * n, 0))) return rc;
+            if ((rc = scale_flat(c, xout + q * ld, X + q * n, 0))) return rc;
         } else {
-            launch_zero(X + r * n, n, s);
+            launch_zero(X + q * n, n, s);
         }
-        bnorm[r] = norm2(s, B + r * n, n, h, part, rc);
+    }
+    for (int r = 0; r < ns; ++r) {
+        bnorm[r] = norm2(s, at(B, r), L.len[r], h, part, rc);
         if (rc) return rc;
     }
-    int total_mv = 0;
     // first: with x = 0 the residual is b itself (no product)
     auto residual = [&](bool first) -> int {
         int q = IFS_OK;
         if (!first || opts->use_x0) {
-            q = matvec_impl(c, X, R, 1, nrhs, n);
+            q = graph ? matvec_graph(c, X, R, 1, nc, n) : matvec_impl(c, X, R, 1, nc, n);
             if (q) return q;
-            ++total_mv;
         }
-        for (int r = 0; r < nrhs; ++r) {
+        for (int r = 0; r < ns; ++r) {
             if (conv[r]) continue;
             if (!first || opts->use_x0) {
                 ++iters[r];
-                launch_axpby(one, B + r * n, mone, R + r * n, n, s);
+                launch_axpby(one, at(B, r), mone, at(R, r), L.len[r], s);
             } else {
-                launch_scale_copy(one, B + r * n, R + r * n, n, s);
+                launch_scale_copy(one, at(B, r), at(R, r), L.len[r], s);
             }
-            beta[r] = norm2(s, R + r * n, n, h, part, q);
+            beta[r] = norm2(s, at(R, r), L.len[r], h, part, q);
             if (q) return q;
             conv[r] = bnorm[r] == 0.0 || beta[r] / bnorm[r] < opts->tol;
-            // the flat-basis residual of this true residual (R of a converged right-hand
-            // side is overwritten by later batched products)
-            flat[r] = bnorm[r] > 0.0 ? flat_rel_residual(c, R + r * n, B + r * n, n) : 0.0;
+            // the flat-basis residual of this true residual (R of a converged system is
+            // overwritten by later products)
+            flat[r] = bnorm[r] > 0.0 ? flat_rel_residual(c, at(R, r), at(B, r), L.len[r]) : 0.0;
         }
         return IFS_OK;
     };
@@ -284,14 +315,14 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
         int jend = 0;
         bool done = false;
     };
-    std::vector<St> st(nrhs);
+    std::vector<St> st(ns);
     auto active = [&](int r) { return !conv[r] && !brk[r] && iters[r] < opts->max_iter; };
     std::vector<cplx> hv, hv2;
     for (;;) {
         bool any = false;
-        for (int r = 0; r < nrhs; ++r) any = any || active(r);
+        for (int r = 0; r < ns; ++r) any = any || active(r);
         if (!any) break;
-        for (int r = 0; r < nrhs; ++r) {
+        for (int r = 0; r < ns; ++r) {
             St &S = st[r];
             S.done = !active(r);
             S.jend = 0;
@@ -301,30 +332,37 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
             S.cs.assign(m, 0.0);
             S.sn.assign(m, 0.0);
             S.g[0] = beta[r];
-            launch_scale_copy(make_double2(1.0 / beta[r], 0.0), R + r * n, V + r * ldv, n, s);
+            launch_scale_copy(make_double2(1.0 / beta[r], 0.0), at(R, r), kv(r, 0), L.len[r], s);
         }
         for (int j = 0; j < m; ++j) {
             bool work = false;
-            for (int r = 0; r < nrhs; ++r) work = work || !st[r].done;
+            for (int r = 0; r < ns; ++r) work = work || !st[r].done;
             if (!work) break;
-            // w_r = M v_{r,j} for every right-hand side in one call
-            if ((rc = matvec_impl(c, V + (int64_t)j * n, W, 1, nrhs, ldv, n, kprec))) return rc;
-            ++total_mv;
-            for (int r = 0; r < nrhs; ++r) {
+            // w = M v_j for every column in one call
+            if (graph) {
+                for (int q = 0; q < nc; ++q) launch_scale_copy(one, V + q * ldv + (int64_t)j * n, stage + q * n, n, s);
+                rc = matvec_graph(c, stage, W, 1, nc, n, n, kprec);
+            } else {
+                rc = matvec_impl(c, V + (int64_t)j * n, W, 1, nc, ldv, n, kprec);
+            }
+            if (rc) return rc;
+            for (int r = 0; r < ns; ++r) {
                 St &S = st[r];
                 if (S.done) continue;
                 ++iters[r];
-                double2 *Vr = V + r * ldv, *w = W + r * n;
-                if ((rc = dots(s, Vr, n, j + 1, w, n, h, part, hv))) return rc;
+                const double2 *Vr = kv(r, 0);
+                double2 *w = at(W, r);
+                const int64_t len = L.len[r];
+                if ((rc = dots(s, Vr, n, j + 1, w, len, h, part, hv))) return rc;
                 if (hipMemcpyAsync(h, hv.data(), sizeof(double2) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess)
                     return IFS_E_HIP;
-                launch_gemv_sub(Vr, n, j + 1, h, w, n, s);
-                if ((rc = dots(s, Vr, n, j + 1, w, n, h, part, hv2))) return rc;
+                launch_gemv_sub(Vr, n, j + 1, h, w, len, s);
+                if ((rc = dots(s, Vr, n, j + 1, w, len, h, part, hv2))) return rc;
                 if (hipMemcpyAsync(h, hv2.data(), sizeof(double2) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess)
                     return IFS_E_HIP;
-                launch_gemv_sub(Vr, n, j + 1, h, w, n, s);
+                launch_gemv_sub(Vr, n, j + 1, h, w, len, s);
                 for (int i = 0; i <= j; ++i) hv[i] += hv2[i];
-                double hn = norm2(s, w, n, h, part, rc);
+                double hn = norm2(s, w, len, h, part, rc);
                 if (rc) return rc;
                 std::vector<cplx> &H = S.H;
                 for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i];
@@ -359,11 +397,19 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
                 } else if (est < opts->tol || iters[r] >= opts->max_iter || j + 1 == m) {
                     S.done = true;
                 } else {
-                    launch_scale_copy(make_double2(1.0 / hn, 0.0), w, Vr + (int64_t)(j + 1) * n, n, s);
+                    launch_scale_copy(make_double2(1.0 / hn, 0.0), w, kv(r, j + 1), len, s);
                 }
             }
+            if (opts->verbose && ns > 1) {
+                int nact = 0;
+                for (int r = 0; r < ns; ++r) nact += !st[r].done;
+                double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::fprintf(stderr, "gmres: %d systems, step %d, %d still in the cycle, %.1f s\n", ns, j + 1, nact,
+                             el);
+                std::fflush(stderr);
+            }
         }
-        for (int r = 0; r < nrhs; ++r) {
+        for (int r = 0; r < ns; ++r) {
             St &S = st[r];
             if (S.jend == 0) continue;
             std::vector<cplx> y(S.jend);
@@ -374,18 +420,18 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
             }
             if (hipMemcpyAsync(h, y.data(), sizeof(double2) * S.jend, hipMemcpyHostToDevice, s) != hipSuccess)
                 return IFS_E_HIP;
-            launch_gemv_add(V + r * ldv, n, S.jend, h, X + r * n, n, s);
+            launch_gemv_add(kv(r, 0), n, S.jend, h, at(X, r), L.len[r], s);
         }
         if ((rc = residual(false))) return rc;
-        for (int r = 0; r < nrhs; ++r)
+        for (int r = 0; r < ns; ++r)
             if (brk[r] && conv[r]) brk[r] = 0;
     }
-    for (int r = 0; r < nrhs; ++r)
-        if ((rc = scale_flat(c, X + r * n, xout + r * ld, 1))) return rc;
+    for (int q = 0; q < nc; ++q)
+        if ((rc = scale_flat(c, X + q * n, xout + q * ld, 1))) return rc;
     if (hipStreamSynchronize(s) != hipSuccess) return IFS_E_HIP;
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int all = 1, anybrk = 0;
-    for (int r = 0; r < nrhs; ++r) {
+    for (int r = 0; r < ns; ++r) {
         if (info) {
             info[r].iterations = iters[r];
             info[r].converged = conv[r];
@@ -396,9 +442,71 @@ extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_
         all = all && conv[r];
         anybrk = anybrk || brk[r];
     }
-    (void)total_mv;
     if (all) return IFS_OK;
     return anybrk ? IFS_E_BREAKDOWN : IFS_E_NOCONV;
+}
+
+int check_opts(const ifs_solve_opts *opts) {
+    if (opts->restart < 1 || opts->max_iter < 1 || !(opts->tol > 0.0)) return IFS_E_ARG;
+    if (opts->precision != IFS_PREC_FP64 && opts->precision != IFS_PREC_MIXED) return IFS_E_ARG;
+    return IFS_OK;
+}
+
+}  // namespace
+
+// nrhs right-hand sides of one cluster in lock step (one system per column).
+extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n,
+                                      int64_t ld, const ifs_solve_opts *opts, ifs_solve_info *info) {
+    if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c) || nrhs < 1 || ld < n) return IFS_E_ARG;
+    if (check_opts(opts)) return IFS_E_ARG;
+    Systems L;
+    L.ncols = nrhs;
+    L.ncol = n;
+    for (int r = 0; r < nrhs; ++r) {
+        L.col.push_back(r);
+        L.off.push_back(0);
+        L.len.push_back(n);
+    }
+    return solve_systems(c, L, (const double2 *)rhs_dev, (double2 *)x_dev, ld, opts, info);
+}
+
+// Every cluster of a multi-cluster context on its own Krylov space, all in one
+// product per Arnoldi step (the operator is block diagonal).
+extern "C" int ifs_solve_multi_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, int64_t n,
+                                      const ifs_solve_opts *opts, ifs_solve_info *info) {
+    if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c)) return IFS_E_ARG;
+    if (check_opts(opts)) return IFS_E_ARG;
+    const int ng = ifs_cluster_count(c);
+    std::vector<int64_t> first(ng + 1);
+    ifs_cluster_offsets(c, nullptr, first.data());
+    Systems L;
+    L.ncols = 1;
+    L.ncol = n;
+    for (int g = 0; g < ng; ++g) {
+        L.col.push_back(0);
+        L.off.push_back(first[g]);
+        L.len.push_back(first[g + 1] - first[g]);
+    }
+    return solve_systems(c, L, (const double2 *)rhs_dev, (double2 *)x_dev, n, opts, info);
+}
+
+extern "C" int ifs_solve_multi(ifs_ctx *c, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
+                               ifs_solve_info *info) {
+    if (!c || !rhs || !x || !opts || n != ifs_vector_length(c)) return IFS_E_ARG;
+    hipStream_t s = ctx_stream(c);
+    DevBuf db, dx;
+    int rc;
+    if ((rc = db.alloc(16ull * n)) || (rc = dx.alloc(16ull * n))) return rc;
+    if (hipMemcpyAsync(db.p, rhs, 16ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (opts->use_x0 && hipMemcpyAsync(dx.p, x, 16ull * n, hipMemcpyHostToDevice, s) != hipSuccess))
+        return IFS_E_HIP;
+    rc = ifs_solve_multi_device(c, db.p, dx.p, n, opts, info);
+    if (rc == IFS_OK || rc == IFS_E_NOCONV || rc == IFS_E_BREAKDOWN) {
+        if (hipMemcpyAsync(x, dx.p, 16ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return IFS_E_HIP;
+    }
+    return rc;
 }
 
 extern "C" int ifs_solve_batch(ifs_ctx *c, int32_t nrhs, const double *rhs, double *x, int64_t n,

@@ -76,6 +76,11 @@ class MultiSphereSolver:
         check(lib().ifs_matvec_device(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(b_ptr), self.n),
               "ifs_matvec_device")
 
+    def calculate_vector_b_graph_device(self, x_ptr, b_ptr):
+        """Same through a HIP graph of the matvec's launches (ifs_matvec_graph_device)."""
+        check(lib().ifs_matvec_graph_device(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(b_ptr), self.n),
+              "ifs_matvec_graph_device")
+
     def get_vector_b(self):
         """Right-hand side for an x-polarised plane wave along +z (lib_mie_ms_solver_get_vector_b_all)."""
         rhs = np.empty(self.n, dtype=np.complex128)
@@ -89,12 +94,20 @@ class MultiSphereSolver:
         check(lib().ifs_plane_wave(self._h, float(alpha), float(beta), p.ctypes.data, self.n), "ifs_plane_wave")
         return p
 
-    def get_vector_b_dir(self, alpha=0.0, beta=0.0):
-        """Right-hand side T p for a plane wave along (alpha, beta)."""
+    def get_vector_b_dir(self, alpha=0.0, beta=0.0, te=False):
+        """Right-hand side T p for a plane wave along (alpha, beta); te: the TE mode (p and
+        q exchanged, lib_mie_illumination.f90:347-352)."""
         rhs = np.empty(self.n, dtype=np.complex128)
-        check(lib().ifs_plane_wave_rhs_dir(self._h, float(alpha), float(beta), rhs.ctypes.data, self.n),
-              "ifs_plane_wave_rhs_dir")
+        check(lib().ifs_plane_wave_rhs_mode(self._h, float(alpha), float(beta), int(bool(te)), rhs.ctypes.data,
+                                            self.n), "ifs_plane_wave_rhs_mode")
         return rhs
+
+    def plane_wave_mode(self, alpha=0.0, beta=0.0, te=False):
+        """Incident coefficients of the TM (te=False) or TE (te=True) wave along (alpha, beta)."""
+        p = np.empty(self.n, dtype=np.complex128)
+        check(lib().ifs_plane_wave_mode(self._h, float(alpha), float(beta), int(bool(te)), p.ctypes.data, self.n),
+              "ifs_plane_wave_mode")
+        return p
 
     def cross_sections(self, x, p):
         """(C_sca, C_ext, C_abs) of the cluster from solved coefficients x and the incident p."""
@@ -144,20 +157,25 @@ class MultiSphereSolver:
         return an, bn
 
     def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True, precision="fp64",
-              verbose=False):
+              verbose=False, graph=False):
         """Restarted GMRES on the device; returns (x, info dict) (lib_mie_ms_solver_run).
         precision 'mixed': FP32 box translations in the Arnoldi products, FP64 true residual
         per cycle (iterative refinement).  verbose: progress lines on stderr."""
         b = self.get_vector_b() if rhs is None else np.ascontiguousarray(rhs, dtype=np.complex128)
         x = np.zeros(self.n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
         op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None), int(bool(verbose)),
-                            _prec(precision))
+                            _prec(precision), _lib.IFS_SOLVE_GRAPH if graph else 0)
         inf = _lib.SolveInfo()
         rc = lib().ifs_solve(self._h, b.ctypes.data, x.ctypes.data, self.n, ctypes.byref(op), ctypes.byref(inf))
         if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
             check(rc, "ifs_solve")
         return x, dict(iterations=inf.iterations, converged=bool(inf.converged), rel_residual=inf.rel_residual,
                        rel_residual_flat=inf.rel_residual_flat, seconds=inf.seconds)
+
+    def calculate_vector_b_batch_device(self, x_ptr, b_ptr, nrhs, ld=None):
+        """b_r = (I + T A) x_r for nrhs device vectors at stride ld (default n), one batched pass."""
+        check(lib().ifs_matvec_batch_device(self._h, int(nrhs), ctypes.c_void_p(x_ptr), ctypes.c_void_p(b_ptr),
+                                            self.n, int(ld or self.n)), "ifs_matvec_batch_device")
 
     def calculate_vector_b_batch(self, X):
         """b_r = (I + T A) x_r for the rows x_r of X [nrhs, n] in one batched pass."""
@@ -183,9 +201,9 @@ class MultiSphereSolver:
                         rel_residual_flat=i.rel_residual_flat, seconds=i.seconds) for i in infos]
 
     def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False, verbose=False,
-                     precision="fp64"):
+                     precision="fp64", graph=False):
         op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)), int(bool(verbose)),
-                            _prec(precision))
+                            _prec(precision), _lib.IFS_SOLVE_GRAPH if graph else 0)
         inf = _lib.SolveInfo()
         rc = lib().ifs_solve_device(self._h, ctypes.c_void_p(rhs_ptr), ctypes.c_void_p(x_ptr), self.n,
                                     ctypes.byref(op), ctypes.byref(inf))
@@ -250,3 +268,79 @@ class MultiSphereSolver:
             self.close()
         except Exception:
             pass
+
+
+class ClusterBatch(MultiSphereSolver):
+    """Many independent clusters in one context (ifs_create_multi): the operator is
+    block diagonal, one matvec launch covers every cluster's sphere pairs, and
+    ``solve`` runs GMRES per cluster in lock step (ifs_solve_multi).  ``clusters`` is a
+    list of dicts with centres, radii, m_rel and lmax (cluster.make_cluster's form);
+    one wavenumber k for all.  Vectors are the clusters' flat vectors one after
+    another (``segment``)."""
+
+    def __init__(self, clusters, k, n_global=None, z_selector=3, workspace_mb=0, cache_mb=0, precision="fp64"):
+        keep, cls = [], []
+        for c in clusters:
+            centres = np.ascontiguousarray(c["centres"], dtype=np.float64).reshape(-1, 3)
+            radii = np.ascontiguousarray(c["radii"], dtype=np.float64)
+            ns = radii.shape[0]
+            m_rel = np.ascontiguousarray(np.broadcast_to(np.asarray(c["m_rel"], dtype=np.complex128), (ns,)))
+            lmax = c.get("lmax")
+            if lmax is None:
+                x = k * radii
+                lmax = np.ceil(x + 4.05 * np.cbrt(x) + 2.0)
+            lmax = np.ascontiguousarray(lmax, dtype=np.int32)
+            keep.append((centres, radii, m_rel, lmax))
+            cls.append(_lib.Cluster(ns, centres.ctypes.data, radii.ctypes.data, m_rel.ctypes.data, lmax.ctypes.data))
+        all_l = np.concatenate([kp[3] for kp in keep])
+        if n_global is None:
+            n_global = int(all_l.max())
+        self._keep = keep
+        arr = (_lib.Cluster * len(cls))(*cls)
+        pr = _lib.Params()
+        pr.k = float(k)
+        pr.n_global = int(n_global)
+        pr.z_selector = int(z_selector)
+        pr.workspace_mb = int(workspace_mb)
+        pr.cache_mb = int(cache_mb)
+        pr.precision = _prec(precision)
+        h = ctypes.c_void_p()
+        check(lib().ifs_create_multi(ctypes.byref(h), len(cls), arr, ctypes.byref(pr)), "ifs_create_multi")
+        self._h = h
+        self.n_spheres = int(all_l.shape[0])
+        self.n_global = int(n_global)
+        self.lmax = all_l
+        self.k = float(k)
+        self.n = int(lib().ifs_vector_length(h))
+        self.n_clusters = int(lib().ifs_cluster_count(h))
+        self.first_sphere = np.empty(self.n_clusters + 1, dtype=np.int64)
+        self.first_entry = np.empty(self.n_clusters + 1, dtype=np.int64)
+        check(lib().ifs_cluster_offsets(h, self.first_sphere.ctypes.data, self.first_entry.ctypes.data),
+              "ifs_cluster_offsets")
+
+    def segment(self, v, g):
+        """Cluster g's entries of a flat vector of this context."""
+        return v[self.first_entry[g]:self.first_entry[g + 1]]
+
+    def solve(self, rhs=None, restart=50, max_iter=1000, tol=1e-6, x0=None, raise_on_noconv=True, precision="fp64",
+              verbose=False, graph=False):
+        """Every cluster by restarted GMRES in lock step; returns (x, [info per cluster])."""
+        b = self.get_vector_b() if rhs is None else np.ascontiguousarray(rhs, dtype=np.complex128)
+        x = np.zeros(self.n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(x0 is not None), int(bool(verbose)),
+                            _prec(precision), _lib.IFS_SOLVE_GRAPH if graph else 0)
+        infos = (_lib.SolveInfo * self.n_clusters)()
+        rc = lib().ifs_solve_multi(self._h, b.ctypes.data, x.ctypes.data, self.n, ctypes.byref(op), infos)
+        if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
+            check(rc, "ifs_solve_multi")
+        return x, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
+                        rel_residual_flat=i.rel_residual_flat, seconds=i.seconds) for i in infos]
+
+    def cross_sections_cluster(self, x, p, g):
+        """C_sca, C_ext, C_abs of cluster g alone (the other clusters' entries zeroed)."""
+        xs = np.zeros_like(x)
+        ps = np.zeros_like(p)
+        sl = slice(self.first_entry[g], self.first_entry[g + 1])
+        xs[sl] = x[sl]
+        ps[sl] = p[sl]
+        return self.cross_sections(xs, ps)

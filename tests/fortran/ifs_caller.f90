@@ -28,7 +28,7 @@ module ifs_bind
     type, bind(C) :: ifs_solve_opts
         integer(c_int32_t) :: restart, max_iter
         real(c_double)     :: tol
-        integer(c_int32_t) :: use_x0, verbose, precision, reserved
+        integer(c_int32_t) :: use_x0, verbose, precision, flags
     end type
 
     type, bind(C) :: ifs_solve_info
