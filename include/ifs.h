@@ -326,6 +326,11 @@ int ifs_profile_read(ifs_ctx *ctx, const char **names, double *total_ms, int64_t
  * and the identity. */
 int ifs_matvec_cost(ifs_ctx *ctx, double *flops, double *bytes);
 
+/* Instrumentation: coaxial-shift launches of the box translations since creation,
+ * counts[0] with 64-wide translation tiles and counts[1] with the 32-wide tile that a
+ * segment remainder of at most 32 translations (columns x right-hand sides) takes. */
+int ifs_class_tile_stats(ifs_ctx *ctx, int64_t counts[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -584,10 +584,18 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
             d.dst_stride = B.dst_stride;
             return P;
         };
-        // rotation runs: consecutive fragments with one polar angle
+        // rotation runs: consecutive fragments with one polar angle and one length.  A run
+        // never crosses an axial segment: with nrhs > 1 a piece lays out its columns as
+        // [right-hand side][translation] from its own first column, so the coaxial shift
+        // (which applies one table to every column of its segment, whatever their order)
+        // reads and writes exactly the columns the runs inside it wrote and read back only
+        // when the runs partition the segment.  (Consecutive classes of one polar angle
+        // and two lengths occur where a level holds few offsets.)
         for (size_t i = 0; i < frags.size();) {
             size_t j = i + 1;
-            while (j < frags.size() && same_d(B.gen[frags[j].first], B.gen[frags[i].first])) ++j;
+            while (j < frags.size() && same_d(B.gen[frags[j].first], B.gen[frags[i].first]) &&
+                   same_ax(B.gen[frags[j].first], B.gen[frags[i].first]))
+                ++j;
             ch.runs.push_back(piece(frags[i].first, frags[i].second.first, frags[j - 1].second.second));
             i = j;
         }
@@ -753,6 +761,13 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
         }
         for (auto &P : ch.segs) {
             need_ax(P);
+            if (P.g.ncols > 0 && P.g.nrhs > 0) {
+                int nfull;
+                bool tail;
+                axial_tile_split(P.g.ncols * P.g.nrhs, nfull, tail);
+                c->axial_tiles[0] += nfull > 0;
+                c->axial_tiles[1] += tail;
+            }
             Scope s(c, pn_ax, fax * P.g.ncols * nrhs);
             launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, prec, ftab, c->stream);
         }
@@ -1174,6 +1189,13 @@ extern "C" {
 int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
     if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
     return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n, -1, c->prm.precision);
+}
+
+int ifs_class_tile_stats(ifs_ctx *c, int64_t counts[2]) {
+    if (!c || !counts) return IFS_E_ARG;
+    counts[0] = c->axial_tiles[0];
+    counts[1] = c->axial_tiles[1];
+    return IFS_OK;
 }
 
 int ifs_matvec_graph_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {

@@ -156,6 +156,7 @@ struct ifs_ctx {
 
     ifs::Profiler prof;
     double flops_mv = 0.0, bytes_mv = 0.0;
+    int64_t axial_tiles[2] = {0, 0};  // coaxial-shift launches: 64-wide tiles, 32-wide remainder tiles
 
     // HIP graphs of the matvec, one per argument set; a graph is replayed while the
     // signature of the buffers and plans it captured still holds

@@ -670,12 +670,14 @@ static void launch_class_t(const GroupDesc &h, const GroupDesc *d_desc, const in
                            wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
-        const int nt = h.ncols * h.nrhs, rem = nt % 64;
-        const int nfull = rem > 0 && rem <= 32 ? nt / 64 : (nt + 63) / 64;  // 64-wide tiles
+        const int nt = h.ncols * h.nrhs;
+        int nfull;
+        bool tail;
+        axial_tile_split(nt, nfull, tail);
         if (nfull > 0)
             hipLaunchKernelGGL((k_class_axial<TC, 64>), dim3(nfull, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax,
                                (const TC *)wy, wz, 0);
-        if (nfull * 64 < nt)  // a remainder of at most 32 translations
+        if (tail)  // a remainder of at most 32 translations
             hipLaunchKernelGGL((k_class_axial<TC, 32>), dim3(1, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax,
                                (const TC *)wy, wz, nfull * 64);
     } else {

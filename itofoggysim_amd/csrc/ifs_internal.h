@@ -182,6 +182,13 @@ void launch_axial_apply(const Task *tasks, int ntasks, int64_t max_rows, const d
                         double2 *wz, hipStream_t s);
 // Coaxial shift Y -> Z with the coefficients generated on the fly (needs the
 // sectorial columns of launch_gen_sect; target orders < 1024 when Ls <= Lt).
+// Coaxial-shift tiles of a segment of nt translation columns: full 64-wide tiles, and one
+// 32-wide tile for a remainder of at most 32 (a larger remainder takes a 64-wide tile).
+inline void axial_tile_split(int nt, int &nfull, bool &tail) {
+    const int rem = nt % 64;
+    nfull = rem > 0 && rem <= 32 ? nt / 64 : (nt + 63) / 64;
+    tail = nfull * 64 < nt;
+}
 void launch_axial_fused(const Task *tasks, int ntasks, int maxK, int max_nmax, int max_Lt, const double2 *wsect,
                         const double2 *wy, double2 *wz, hipStream_t s);
 void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblocks, const double *wd,

@@ -244,6 +244,12 @@ class MultiSphereSolver:
         check(lib().ifs_matvec_cost(self._h, ctypes.byref(f), ctypes.byref(b)), "ifs_matvec_cost")
         return f.value, b.value
 
+    def class_tile_stats(self):
+        """(64-wide, 32-wide remainder) coaxial-shift launches of the box translations so far."""
+        t = (ctypes.c_int64 * 2)()
+        check(lib().ifs_class_tile_stats(self._h, ctypes.cast(t, ctypes.c_void_p)), "ifs_class_tile_stats")
+        return int(t[0]), int(t[1])
+
     def fmm_stats(self):
         counts = (ctypes.c_int64 * 6)()
         lv = (ctypes.c_int32 * 32)()

@@ -161,3 +161,34 @@ def test_fmm_vs_direct_odd_tiles_batched(n, leaf, d0):
         assert e1 < 1e-7 and eb < 1e-7
     f.close()
     d.close()
+
+
+def test_fmm_batched_tail_tiles_vs_oracle_fmm(oracle):
+    """Three right-hand sides in one batched MLFMM matvec against the oracle's MLFMM
+    restatement, 1e-10, on trees whose coaxial-shift segments hold more than 64
+    translation columns (columns x right-hand sides) with a remainder of at most 32:
+    both the 64-wide tiles and the 32-wide remainder tile run (ifs_class_tile_stats)."""
+    from oracle import fmm_ref
+    tails = 0
+    for ns, seed in ((40, 12), (64, 21)):
+        rng = np.random.default_rng(seed)
+        pos = rng.uniform(0.0, 8.0, (ns, 3))
+        lmax = rng.integers(1, 4, ns).astype(np.int32)
+        k = 0.6
+        s = MultiSphereSolver(pos, np.full(ns, 0.2), 1.5 + 0.01j, k, lmax=lmax, use_fmm=True, leaf_size=1,
+                              fmm_d0=3.0)
+        an, bn = s.mie_coefficients()
+        X = np.stack([_rand(s.n, seed + r) for r in range(3)])
+        t0 = s.class_tile_stats()
+        B = s.calculate_vector_b_batch(X)
+        t1 = s.class_tile_stats()
+        full, tail = t1[0] - t0[0], t1[1] - t0[1]
+        for r in range(3):
+            ref, _ = fmm_ref.fmm_matvec(pos, lmax, s.n_global, an, bn, k, X[r], s=1, d0=3.0, cap=0, dsign=-1.0)
+            err = np.linalg.norm(B[r] - ref) / np.linalg.norm(ref - X[r])
+            print(f"ns={ns} rhs {r}: batched vs oracle FMM {err:.2e}; 64-wide launches {full}, 32-wide {tail}")
+            assert err < 1e-10
+        assert full > 0
+        tails += tail
+        s.close()
+    assert tails > 0
