@@ -330,6 +330,12 @@ int ifs_matvec_cost(ifs_ctx *ctx, double *flops, double *bytes);
  * counts[0] with 64-wide translation tiles and counts[1] with the 32-wide tile that a
  * segment remainder of at most 32 translations (columns x right-hand sides) takes. */
 int ifs_class_tile_stats(ifs_ctx *ctx, int64_t counts[2]);
+/* Block heights of the box translations' rotation products: 64 or 32 rows (the same
+ * results), or 0 to let the autotuner choose.  With autotune = 1 (the default) a
+ * batch's first product times both heights on every launch and keeps the faster per
+ * product; choices are remembered per shape for the rest of the process.  Drops the
+ * context's captured graphs. */
+int ifs_set_class_tiles(ifs_ctx *ctx, int32_t fwd_rows, int32_t back_rows, int32_t autotune);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,7 @@ SIGNATURES = {
     "ifs_tree_build_gpu": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
     "ifs_matvec_graph_device": (_i32, [_vp, _vp, _vp, _i64]),
     "ifs_class_tile_stats": (_i32, [_vp, _vp]),
+    "ifs_set_class_tiles": (_i32, [_vp, _i32, _i32, _i32]),
     "ifs_plane_wave_mode": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _i32, _vp, _i64]),
     "ifs_plane_wave_rhs_mode": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _i32, _vp, _i64]),
     "ifs_create_multi": (_i32, [ctypes.POINTER(_vp), _i32, ctypes.POINTER(Cluster), ctypes.POINTER(Params)]),

@@ -9,11 +9,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -690,6 +692,28 @@ int class_batch_plan(ifs_ctx *c, ClassBatch &B, int nrhs) {
     return IFS_OK;
 }
 
+// Rotation block heights chosen by timing, per (Ls, Lt, K, precision, right-hand sides,
+// log2 of the translation count), shared by every context of the process.
+static std::mutex g_rows_mu;
+static std::map<std::array<int, 6>, std::array<int, 2>> g_rows;
+static int ilog2(int64_t v) {
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+static int tuned_rows(const std::array<int, 6> &k, int st) {
+    std::lock_guard<std::mutex> lk(g_rows_mu);
+    auto it = g_rows.find(k);
+    return it == g_rows.end() ? 0 : it->second[st];
+}
+static void remember_rows(const std::array<int, 6> &k, int st, int r) {
+    std::lock_guard<std::mutex> lk(g_rows_mu);
+    g_rows[k][st] = r;
+}
+
 int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 *dst_pool, int nrhs, int prec) {
     if (B.gen.empty() || nrhs <= 0) return IFS_OK;
     if (B.plan_nrhs != nrhs) {
@@ -753,11 +777,45 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
     const char *pn_fwd = prec == 1 ? "class_rot_fwd_f32" : "class_rot_fwd";
     const char *pn_ax = prec == 1 ? "class_axial_f32" : "class_axial";
     const char *pn_back = prec == 1 ? "class_rot_back_f32" : "class_rot_back";
+    // Rotation block heights (64 or 32 rows of each folded product).  64 halves the
+    // column-panel re-reads; 32 wastes less padding where the degrees hold few rows
+    // (small orders) and gives more blocks to short runs.  A batch's first product
+    // times both on every run launch (they write the same Y or W), keeps the faster per
+    // product, and remembers the choice per shape for later contexts.
+    int rows[2];
+    bool tune[2] = {false, false};
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(c->stream, &cap);
+    const bool can_tune = c->autotune && !c->prof.on && cap == hipStreamCaptureStatusNone;
+    const std::array<int, 6> tkey = {T0.Ls, T0.Lt, T0.K, prec, nrhs, ilog2((int64_t)B.src.size())};
+    for (int st = 0; st < 2; ++st) {
+        int r = c->force_rows[st] ? c->force_rows[st] : B.rot_rows[prec & 1][st];
+        if (!r) r = tuned_rows(tkey, st);
+        if (!r && can_tune) tune[st] = true;
+        rows[st] = r ? r : 64;
+    }
+    std::vector<std::array<hipEvent_t, 3>> tev[2];
+    auto launch_rot = [&](const ClassBatch::Piece &P, int st) {
+        const int stage = st == 0 ? 0 : 2;
+        if (!tune[st]) {
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, stage, prec, ftab, rows[st],
+                         c->stream);
+            return;
+        }
+        std::array<hipEvent_t, 3> e{};
+        for (auto &x : e) (void)hipEventCreate(&x);
+        (void)hipEventRecord(e[0], c->stream);
+        launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, stage, prec, ftab, 64, c->stream);
+        (void)hipEventRecord(e[1], c->stream);
+        launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, stage, prec, ftab, 32, c->stream);
+        (void)hipEventRecord(e[2], c->stream);
+        tev[st].push_back(e);
+    };
     for (auto &ch : B.chunks) {
         for (auto &P : ch.runs) {
             need_d(P);
             Scope s(c, pn_fwd, ffwd * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 0, prec, ftab, c->stream);
+            launch_rot(P, 0);
         }
         for (auto &P : ch.segs) {
             need_ax(P);
@@ -769,13 +827,13 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
                 c->axial_tiles[1] += tail;
             }
             Scope s(c, pn_ax, fax * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, prec, ftab, c->stream);
+            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 1, prec, ftab, 0, c->stream);
         }
         for (size_t r = ch.runs.size(); r-- > 0;) {
             auto &P = ch.runs[r];
             need_d(P);
             Scope s(c, pn_back, fback * P.g.ncols * nrhs);
-            launch_class(P.g, dd + P.di, dsrc, dpidx, dph, src_pool, wd, wax, wy, wz, 2, prec, ftab, c->stream);
+            launch_rot(P, 1);
         }
         {
             Scope s(c, "class_scatter");
@@ -785,6 +843,24 @@ int class_batch_run(ifs_ctx *c, ClassBatch &B, const double2 *src_pool, double2 
         }
     }
     if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
+    for (int st = 0; st < 2; ++st) {
+        if (!tune[st]) continue;
+        double t64 = 0.0, t32 = 0.0;
+        bool ok = !tev[st].empty() && hipEventSynchronize(tev[st].back()[2]) == hipSuccess;
+        for (auto &e : tev[st]) {
+            float a = 0.f, b = 0.f;
+            ok = ok && hipEventElapsedTime(&a, e[0], e[1]) == hipSuccess &&
+                 hipEventElapsedTime(&b, e[1], e[2]) == hipSuccess;
+            t64 += a;
+            t32 += b;
+            for (auto &x : e) (void)hipEventDestroy(x);
+        }
+        if (!ok) continue;
+        // 32 rows only when clearly faster (the two agree to within noise on large orders)
+        const int r = t32 < 0.97 * t64 ? 32 : 64;
+        B.rot_rows[prec & 1][st] = r;
+        remember_rows(tkey, st, r);
+    }
     return IFS_OK;
 }
 
@@ -1189,6 +1265,16 @@ extern "C" {
 int ifs_matvec_device(ifs_ctx *c, const void *x_dev, void *b_dev, int64_t n) {
     if (!c || !x_dev || !b_dev || n != c->nflat) return IFS_E_ARG;
     return matvec_impl(c, (const double2 *)x_dev, (double2 *)b_dev, 0, 1, n, -1, c->prm.precision);
+}
+
+int ifs_set_class_tiles(ifs_ctx *c, int32_t fwd_rows, int32_t back_rows, int32_t autotune) {
+    auto ok = [](int32_t r) { return r == 0 || r == 32 || r == 64; };
+    if (!c || !ok(fwd_rows) || !ok(back_rows) || (autotune != 0 && autotune != 1)) return IFS_E_ARG;
+    c->force_rows[0] = fwd_rows;
+    c->force_rows[1] = back_rows;
+    c->autotune = autotune;
+    graphs_clear(c);  // captured launches carry the old block heights
+    return IFS_OK;
 }
 
 int ifs_class_tile_stats(ifs_ctx *c, int64_t counts[2]) {

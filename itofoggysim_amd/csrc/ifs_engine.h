@@ -157,6 +157,8 @@ struct ifs_ctx {
     ifs::Profiler prof;
     double flops_mv = 0.0, bytes_mv = 0.0;
     int64_t axial_tiles[2] = {0, 0};  // coaxial-shift launches: 64-wide tiles, 32-wide remainder tiles
+    int force_rows[2] = {0, 0};       // rotation block heights set by ifs_set_class_tiles (0: autotuned)
+    int autotune = 1;                 // time both rotation block heights on a batch's first product
 
     // HIP graphs of the matvec, one per argument set; a graph is replayed while the
     // signature of the buffers and plans it captured still holds
@@ -217,6 +219,7 @@ struct ClassBatch {
     std::vector<char> d_ready, ax_ready;
     bool d_persist = false, ax_persist = false;
     DevBuf own_d, own_ax, d_dkeys;
+    int rot_rows[2][2] = {{0, 0}, {0, 0}};  // [precision][forward, back]: rotation block height, 0 = not tuned
     int64_t src_stride = 0, dst_stride = 0;  // pool stride between right-hand sides (complex)
     DevBuf d_desc, d_gen, d_src, d_pidx, d_phase, d_scat, d_scat_cols;
     int64_t n_d = 0, n_s = 0, n_ax = 0, n_y = 0, n_z = 0;  // workspace (doubles for d, y, z; complex for s, ax)

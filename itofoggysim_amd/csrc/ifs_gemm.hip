@@ -660,14 +660,19 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterSeg *__restrict__ 
 template <typename TC, typename TW>
 static void launch_class_t(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
                            const double2 *phase, const double2 *src_pool, const TW *wd, const double2 *wax,
-                           TC *wy, TC *wz, int stage, hipStream_t s) {
+                           TC *wy, TC *wz, int stage, int rows, hipStream_t s) {
     const int NC = 4 * h.ncols * h.nrhs;
     const int K = h.K;
     if (stage == 0) {
-        int mt = ((h.Ls < K ? h.Ls : K) + 1 + FMF - 1) / FMF;
+        const int rm = rows == 32 ? 32 : FMF;
+        int mt = ((h.Ls < K ? h.Ls : K) + 1 + rm - 1) / rm;
         dim3 grid((NC + FNF2 - 1) / FNF2, mt, h.Ls);
-        hipLaunchKernelGGL((k_class_rot_fwd<TC, TW, FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase, src_pool,
-                           wd, wy);
+        if (rm == 32)
+            hipLaunchKernelGGL((k_class_rot_fwd<TC, TW, 32, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase,
+                               src_pool, wd, wy);
+        else
+            hipLaunchKernelGGL((k_class_rot_fwd<TC, TW, FMF, 3>), grid, dim3(256), 0, s, d_desc, src_off, pidx, phase,
+                               src_pool, wd, wy);
     } else if (stage == 1) {
         int mt = (h.Lt + AM - 1) / AM;
         const int nt = h.ncols * h.nrhs;
@@ -681,26 +686,31 @@ static void launch_class_t(const GroupDesc &h, const GroupDesc *d_desc, const in
             hipLaunchKernelGGL((k_class_axial<TC, 32>), dim3(1, mt, 2 * K + 1), dim3(256), 0, s, d_desc, wax,
                                (const TC *)wy, wz, nfull * 64);
     } else {
-        int mt = (h.Lt + 1 + FMB - 1) / FMB;
+        const int fm = rows == 32 ? 32 : FMB;
+        int mt = (h.Lt + 1 + fm - 1) / fm;
         dim3 grid((NC + FNB - 1) / FNB, mt, h.Lt);
         // W (FP64 columns) aliases the Y workspace, free once the coaxial shifts are done
-        hipLaunchKernelGGL((k_class_rot_back<TC, TW, FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd,
-                           (const TC *)wz, (double2 *)wy);
+        if (fm == 32)
+            hipLaunchKernelGGL((k_class_rot_back<TC, TW, 32, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd,
+                               (const TC *)wz, (double2 *)wy);
+        else
+            hipLaunchKernelGGL((k_class_rot_back<TC, TW, FMB, 3>), grid, dim3(256), 0, s, d_desc, pidx, phase, wd,
+                               (const TC *)wz, (double2 *)wy);
     }
 }
 
 void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
                   const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
-                  double *wz, int stage, int prec, int ftab, hipStream_t s) {
+                  double *wz, int stage, int prec, int ftab, int rows, hipStream_t s) {
     if (h.ncols <= 0 || h.nrhs <= 0) return;
     if (prec == 1 && ftab)
         launch_class_t<float, float>(h, d_desc, src_off, pidx, phase, src_pool, (const float *)wd, wax, (float *)wy,
-                                     (float *)wz, stage, s);
+                                     (float *)wz, stage, rows, s);
     else if (prec == 1)
         launch_class_t<float, double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, (float *)wy, (float *)wz,
-                                      stage, s);
+                                      stage, rows, s);
     else
-        launch_class_t<double, double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, wy, wz, stage, s);
+        launch_class_t<double, double>(h, d_desc, src_off, pidx, phase, src_pool, wd, wax, wy, wz, stage, rows, s);
 }
 
 void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,

@@ -201,9 +201,11 @@ void launch_rot_back(const Task *tasks, const Segment *segs, int nsegs, int yblo
 // stage 0: forward rotation of a run (source gather with exp(i m phi)) -> Y
 // stage 1: coaxial shift of a segment Y -> Z
 // stage 2: back rotation of a run (exp(-i m phi)) Z -> W (aliases Y)
+// rows: block height of the rotation products, 64 or 32 (the same results; the faster
+// one depends on the orders and the column count, see the autotuner in class_batch_run)
 void launch_class(const GroupDesc &h, const GroupDesc *d_desc, const int64_t *src_off, const int32_t *pidx,
                   const double2 *phase, const double2 *src_pool, const double *wd, const double2 *wax, double *wy,
-                  double *wz, int stage, int prec, int ftab, hipStream_t s);
+                  double *wz, int stage, int prec, int ftab, int rows, hipStream_t s);
 // dst_pool[r dst_stride + seg.dst + e] += sum_{i in [beg,end)} W[r ncy + cols[i]][e],
 // e < nbox = 2 Lt(Lt+2), r < nrhs
 void launch_scatter(const ScatterSeg *segs, int nsegs, const int32_t *cols, const double2 *W, int64_t nbox,

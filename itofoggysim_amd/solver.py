@@ -244,6 +244,11 @@ class MultiSphereSolver:
         check(lib().ifs_matvec_cost(self._h, ctypes.byref(f), ctypes.byref(b)), "ifs_matvec_cost")
         return f.value, b.value
 
+    def set_class_tiles(self, fwd_rows=0, back_rows=0, autotune=True):
+        """Rotation block heights of the box translations (64, 32, or 0 = autotuned)."""
+        check(lib().ifs_set_class_tiles(self._h, int(fwd_rows), int(back_rows), int(bool(autotune))),
+              "ifs_set_class_tiles")
+
     def class_tile_stats(self):
         """(64-wide, 32-wide remainder) coaxial-shift launches of the box translations so far."""
         t = (ctypes.c_int64 * 2)()

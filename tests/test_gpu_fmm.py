@@ -192,3 +192,29 @@ def test_fmm_batched_tail_tiles_vs_oracle_fmm(oracle):
         tails += tail
         s.close()
     assert tails > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_rotation_block_heights(precision):
+    """The rotation products in 64- and 32-row blocks (forced with ifs_set_class_tiles,
+    in every combination for the forward and back products) and as autotuned give the
+    same matvec (each output entry sums its inner terms in the same order)."""
+    c = config_cluster(2, n=200)
+    s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=c["lmax"], use_fmm=True, leaf_size=16,
+                          fmm_d0=6.0, precision=precision)
+    try:
+        S, mask = snm_flat(s.n_global, c["lmax"])
+        rng = np.random.default_rng(99)
+        x = (rng.standard_normal(s.n) + 1j * rng.standard_normal(s.n)) * mask / S
+        s.set_class_tiles(64, 64, autotune=False)
+        ref = s.calculate_vector_b(x) * S
+        den = np.linalg.norm(ref - x * S)
+        for fr, br in ((32, 32), (32, 64), (64, 32), (0, 0)):
+            s.set_class_tiles(fr, br, autotune=True)
+            for rep in range(2):  # the autotuned case tunes on the first product
+                b = s.calculate_vector_b(x) * S
+                e = np.linalg.norm(b - ref) / den
+                print(f"{precision}: forward {fr or 'auto'} rows, back {br or 'auto'} rows, call {rep}: {e:.1e}")
+                assert e < 1e-14
+    finally:
+        s.close()
