@@ -224,6 +224,16 @@ int ifs_solve_batch_device(ifs_ctx *ctx, int32_t nrhs, const void *rhs_dev, void
 int ifs_solve_batch(ifs_ctx *ctx, int32_t nrhs, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
                     ifs_solve_info *info);
 
+/* nrhs systems (I + T A) x_r = rhs_r by restarted block GMRES: one Krylov space shared
+ * by all right-hand sides (restart = block steps per cycle, m nrhs basis vectors; one
+ * batched product per step), so each right-hand side is corrected from every direction
+ * the others explored.  info[r].iterations counts the products per right-hand side.
+ * No reference counterpart (the reference solves one right-hand side per run). */
+int ifs_solve_block_device(ifs_ctx *ctx, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n, int64_t ld,
+                           const ifs_solve_opts *opts, ifs_solve_info *info);
+int ifs_solve_block(ifs_ctx *ctx, int32_t nrhs, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,
+                    ifs_solve_info *info);
+
 /* Many independent clusters in one context: the spheres of clusters[0..nclusters)
  * in order, each cluster interacting only with itself (a block-diagonal operator), so
  * that one matvec launch covers every cluster's pairs.  Direct plans only

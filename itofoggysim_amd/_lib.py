@@ -116,6 +116,9 @@ SIGNATURES = {
     "ifs_cluster_count": (_i32, [_vp]),
     "ifs_cluster_offsets": (_i32, [_vp, _vp, _vp]),
     "ifs_solve_multi_device": (_i32, [_vp, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "ifs_solve_block_device": (_i32, [_vp, _i32, _vp, _vp, _i64, _i64, ctypes.POINTER(SolveOpts),
+                                      ctypes.POINTER(SolveInfo)]),
+    "ifs_solve_block": (_i32, [_vp, _i32, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "ifs_solve_multi": (_i32, [_vp, _vp, _vp, _i64, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "ifs_tree_i4_gpu": (_i32, [_vp, _i64, _i32, _vp, _vp]),
 }

@@ -454,6 +454,246 @@ int check_opts(const ifs_solve_opts *opts) {
 
 }  // namespace
 
+// Block GMRES (restarted): the nrhs right-hand sides share one Krylov space,
+//   K_m(M, R0) = span{R0, M R0, ..., M^{m-1} R0},  R0 = B - M X0  (n x p blocks),
+// so every block step applies one batched product to p Krylov vectors and every right-
+// hand side's iterate is corrected from all m p basis vectors.  Block Arnoldi with
+// classical Gram-Schmidt against the basis (two passes) and modified Gram-Schmidt with
+// re-orthogonalisation inside the new block gives the block Hessenberg matrix H (upper
+// triangular p x p sub-diagonal blocks); Givens rotations reduce it column by column
+// and minimise ||E1 S - H Y||_F for all right-hand sides together, whose column norms
+// of the rotated residual block are the per-right-hand-side residual estimates.
+// A direction that becomes dependent on the basis (norm below 1e-14 of the block's
+// largest) is dropped: its basis vector is zero and contributes nothing.  The true
+// residual of every restart cycle is recomputed with FP64 products.
+static int solve_block(ifs_ctx *c, int p, const double2 *bin, double2 *xout, int64_t ld, const ifs_solve_opts *opts,
+                       ifs_solve_info *info) {
+    const int m = opts->restart;
+    const int kprec = opts->precision == IFS_PREC_MIXED ? 1 : 0;
+    const int64_t n = ifs_vector_length(c);
+    const int mp = m * p, mp1 = (m + 1) * p;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = ctx_stream(c);
+    DevBuf dV, dW, dR, dB, dX, dpart, dh;
+    int rc;
+    if ((rc = dV.alloc(16ull * n * mp1)) || (rc = dW.alloc(16ull * n * p)) || (rc = dR.alloc(16ull * n * p)) ||
+        (rc = dB.alloc(16ull * n * p)) || (rc = dX.alloc(16ull * n * p)) ||
+        (rc = dpart.alloc(16ull * kDotBlocks * mp1)) || (rc = dh.alloc(16ull * mp1)))
+        return rc;
+    double2 *V = dV.as<double2>(), *W = dW.as<double2>(), *R = dR.as<double2>(), *B = dB.as<double2>(),
+            *X = dX.as<double2>(), *part = dpart.as<double2>(), *h = dh.as<double2>();
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
+    std::vector<double> bnorm(p), beta(p), flat(p, 0.0);
+    for (int q = 0; q < p; ++q) {
+        if ((rc = scale_flat(c, bin + q * ld, B + q * n, 0))) return rc;
+        if (opts->use_x0) {
+            if ((rc = scale_flat(c, xout + q * ld, X + q * n, 0))) return rc;
+        } else {
+            launch_zero(X + q * n, n, s);
+        }
+        bnorm[q] = norm2(s, B + q * n, n, h, part, rc);
+        if (rc) return rc;
+    }
+    auto upload = [&](const std::vector<cplx> &v, int cnt) -> int {
+        return hipMemcpyAsync(h, v.data(), sizeof(double2) * cnt, hipMemcpyHostToDevice, s) == hipSuccess ? IFS_OK
+                                                                                                            : IFS_E_HIP;
+    };
+    // orthonormalises the p columns at Q (stride n) among themselves (MGS, two passes);
+    // T[q' + p q] receives the upper-triangular factor (column-major p x p)
+    std::vector<cplx> one_h(1);
+    auto block_qr = [&](double2 *Q, std::vector<cplx> &T) -> int {
+        T.assign((size_t)p * p, 0.0);
+        std::vector<double> nrm0(p);
+        double big = 0.0;
+        for (int q = 0; q < p; ++q) {
+            int e = 0;
+            nrm0[q] = norm2(s, Q + q * n, n, h, part, e);
+            if (e) return e;
+            big = std::max(big, nrm0[q]);
+        }
+        for (int q = 0; q < p; ++q) {
+            double2 *w = Q + q * n;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int r = 0; r < q; ++r) {
+                    std::vector<cplx> d;
+                    int e = dots(s, Q + r * n, n, 1, w, n, h, part, d);
+                    if (e) return e;
+                    T[r + (size_t)p * q] += d[0];
+                    launch_axpby(make_double2(-d[0].real(), -d[0].imag()), Q + r * n, one, w, n, s);
+                }
+            int e = 0;
+            double nr = norm2(s, w, n, h, part, e);
+            if (e) return e;
+            if (nr > 1e-14 * big && nr > 0.0) {
+                T[q + (size_t)p * q] = nr;
+                launch_scale_copy(make_double2(1.0 / nr, 0.0), w, w, n, s);
+            } else {  // dependent direction: dropped
+                T[q + (size_t)p * q] = 0.0;
+                launch_zero(w, n, s);
+            }
+        }
+        return IFS_OK;
+    };
+    int iters = 0, cycles = 0;
+    std::vector<int> conv(p, 0);
+    bool first = true;
+    for (;;) {
+        // true residual R = B - M X in FP64
+        if (!first || opts->use_x0) {
+            if ((rc = matvec_impl(c, X, R, 1, p, n))) return rc;
+            for (int q = 0; q < p; ++q) launch_axpby(one, B + q * n, mone, R + q * n, n, s);
+            if (!first) ++iters;
+        } else {
+            for (int q = 0; q < p; ++q) launch_scale_copy(one, B + q * n, R + q * n, n, s);
+        }
+        first = false;
+        bool all = true;
+        for (int q = 0; q < p; ++q) {
+            beta[q] = norm2(s, R + q * n, n, h, part, rc);
+            if (rc) return rc;
+            conv[q] = bnorm[q] == 0.0 || beta[q] / bnorm[q] < opts->tol;
+            all = all && conv[q];
+            flat[q] = bnorm[q] > 0.0 ? flat_rel_residual(c, R + q * n, B + q * n, n) : 0.0;
+        }
+        if (opts->verbose) {
+            double worst = 0.0;
+            for (int q = 0; q < p; ++q) worst = std::max(worst, bnorm[q] > 0.0 ? beta[q] / bnorm[q] : 0.0);
+            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::fprintf(stderr, "block gmres: %d products, worst true rel residual %.3e, %.1f s\n", iters, worst, el);
+            std::fflush(stderr);
+        }
+        if (all || iters >= opts->max_iter) break;
+        ++cycles;
+        // V_0 S = R
+        if (hipMemcpyAsync(V, R, 16ull * n * p, hipMemcpyDeviceToDevice, s) != hipSuccess) return IFS_E_HIP;
+        std::vector<cplx> S;
+        if ((rc = block_qr(V, S))) return rc;
+        // H: mp1 x mp (column-major), G: mp1 x p (column-major), Givens rotations in order
+        std::vector<cplx> H((size_t)mp1 * mp, 0.0), G((size_t)mp1 * p, 0.0);
+        for (int q = 0; q < p; ++q)
+            for (int r = 0; r <= q; ++r) G[r + (size_t)mp1 * q] = S[r + (size_t)p * q];
+        struct Rot {
+            int k, i;
+            double cs;
+            cplx sn;
+        };
+        std::vector<Rot> rots;
+        auto rot_apply = [&](const Rot &g, cplx &a, cplx &b) {
+            const cplx x = a, y = b;
+            a = g.cs * x + g.sn * y;
+            b = -std::conj(g.sn) * x + g.cs * y;
+        };
+        int jend = 0;
+        for (int j = 0; j < m; ++j) {
+            // W = M V_j (p columns, one product)
+            if ((rc = matvec_impl(c, V + (int64_t)j * p * n, W, 1, p, n, n, kprec))) return rc;
+            ++iters;
+            const int nb = (j + 1) * p;  // basis vectors so far
+            for (int q = 0; q < p; ++q) {
+                double2 *w = W + q * n;
+                cplx *hc = &H[(size_t)mp1 * (j * p + q)];
+                for (int pass = 0; pass < 2; ++pass) {
+                    std::vector<cplx> d;
+                    if ((rc = dots(s, V, n, nb, w, n, h, part, d))) return rc;
+                    if ((rc = upload(d, nb))) return rc;
+                    launch_gemv_sub(V, n, nb, h, w, n, s);
+                    for (int i = 0; i < nb; ++i) hc[i] += d[i];
+                }
+            }
+            // V_{j+1} T = W
+            std::vector<cplx> T;
+            if ((rc = block_qr(W, T))) return rc;
+            if (hipMemcpyAsync(V + (int64_t)nb * n, W, 16ull * n * p, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return IFS_E_HIP;
+            for (int q = 0; q < p; ++q)
+                for (int r = 0; r <= q; ++r) H[(size_t)mp1 * (j * p + q) + nb + r] = T[r + (size_t)p * q];
+            // reduce the new block column: every earlier rotation on all p new columns first,
+            // then the new ones below the diagonal, each applied to the columns it reaches
+            const size_t nold = rots.size();
+            for (int q = 0; q < p; ++q) {
+                cplx *hc = &H[(size_t)mp1 * (j * p + q)];
+                for (size_t g = 0; g < nold; ++g) rot_apply(rots[g], hc[rots[g].k], hc[rots[g].i]);
+            }
+            for (int q = 0; q < p; ++q) {
+                cplx *hc = &H[(size_t)mp1 * (j * p + q)];
+                const int k = j * p + q;
+                for (int i = k + 1; i <= nb + q; ++i) {
+                    const cplx a = hc[k], b = hc[i];
+                    const double ra = std::abs(a), rb = std::abs(b);
+                    if (rb == 0.0) continue;
+                    const double rr = std::hypot(ra, rb);
+                    Rot g{k, i, 0.0, 0.0};
+                    if (ra == 0.0) {
+                        g.cs = 0.0;
+                        g.sn = std::conj(b) / rb;
+                    } else {
+                        g.cs = ra / rr;
+                        g.sn = (a / ra) * std::conj(b) / rr;
+                    }
+                    for (int q2 = q; q2 < p; ++q2) {
+                        cplx *h2 = &H[(size_t)mp1 * (j * p + q2)];
+                        rot_apply(g, h2[k], h2[i]);
+                    }
+                    hc[i] = 0.0;
+                    for (int r = 0; r < p; ++r) rot_apply(g, G[(size_t)mp1 * r + k], G[(size_t)mp1 * r + i]);
+                    rots.push_back(g);
+                }
+            }
+            jend = j + 1;
+            // residual estimate per right-hand side: rows nb .. nb + p of the rotated G
+            bool done = true;
+            for (int r = 0; r < p; ++r) {
+                double e = 0.0;
+                for (int i = nb; i < nb + p; ++i) e += std::norm(G[(size_t)mp1 * r + i]);
+                done = done && (bnorm[r] == 0.0 || std::sqrt(e) / bnorm[r] < opts->tol);
+            }
+            if (opts->verbose) {
+                double worst = 0.0;
+                for (int r = 0; r < p; ++r) {
+                    double e = 0.0;
+                    for (int i = nb; i < nb + p; ++i) e += std::norm(G[(size_t)mp1 * r + i]);
+                    worst = std::max(worst, bnorm[r] > 0.0 ? std::sqrt(e) / bnorm[r] : 0.0);
+                }
+                double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::fprintf(stderr, "block gmres: %d products, worst estimated rel residual %.3e, %.1f s\n", iters,
+                             worst, el);
+                std::fflush(stderr);
+            }
+            if (done || iters >= opts->max_iter) break;
+        }
+        // Y = R^-1 G (upper triangular jend p), X_r += V Y_r
+        const int nk = jend * p;
+        for (int r = 0; r < p; ++r) {
+            std::vector<cplx> y(nk, 0.0);
+            for (int i = nk - 1; i >= 0; --i) {
+                cplx acc = G[(size_t)mp1 * r + i];
+                for (int k = i + 1; k < nk; ++k) acc -= H[(size_t)mp1 * k + i] * y[k];
+                const cplx d = H[(size_t)mp1 * i + i];
+                y[i] = std::abs(d) > 0.0 ? acc / d : cplx(0.0);
+            }
+            if ((rc = upload(y, nk))) return rc;
+            launch_gemv_add(V, n, nk, h, X + r * n, n, s);
+        }
+    }
+    for (int q = 0; q < p; ++q)
+        if ((rc = scale_flat(c, X + q * n, xout + q * ld, 1))) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return IFS_E_HIP;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int all = 1;
+    for (int q = 0; q < p; ++q) {
+        if (info) {
+            info[q].iterations = iters;
+            info[q].converged = conv[q];
+            info[q].rel_residual = bnorm[q] > 0.0 ? beta[q] / bnorm[q] : 0.0;
+            info[q].seconds = secs;
+            info[q].rel_residual_flat = flat[q];
+        }
+        all = all && conv[q];
+    }
+    (void)cycles;
+    return all ? IFS_OK : IFS_E_NOCONV;
+}
+
 // nrhs right-hand sides of one cluster in lock step (one system per column).
 extern "C" int ifs_solve_batch_device(ifs_ctx *c, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n,
                                       int64_t ld, const ifs_solve_opts *opts, ifs_solve_info *info) {
@@ -488,6 +728,33 @@ extern "C" int ifs_solve_multi_device(ifs_ctx *c, const void *rhs_dev, void *x_d
         L.len.push_back(first[g + 1] - first[g]);
     }
     return solve_systems(c, L, (const double2 *)rhs_dev, (double2 *)x_dev, n, opts, info);
+}
+
+// nrhs right-hand sides of one cluster by block GMRES (one shared Krylov space).
+extern "C" int ifs_solve_block_device(ifs_ctx *c, int32_t nrhs, const void *rhs_dev, void *x_dev, int64_t n,
+                                      int64_t ld, const ifs_solve_opts *opts, ifs_solve_info *info) {
+    if (!c || !rhs_dev || !x_dev || !opts || n != ifs_vector_length(c) || nrhs < 1 || ld < n) return IFS_E_ARG;
+    if (check_opts(opts)) return IFS_E_ARG;
+    return solve_block(c, nrhs, (const double2 *)rhs_dev, (double2 *)x_dev, ld, opts, info);
+}
+
+extern "C" int ifs_solve_block(ifs_ctx *c, int32_t nrhs, const double *rhs, double *x, int64_t n,
+                               const ifs_solve_opts *opts, ifs_solve_info *info) {
+    if (!c || !rhs || !x || !opts || n != ifs_vector_length(c) || nrhs < 1) return IFS_E_ARG;
+    hipStream_t s = ctx_stream(c);
+    DevBuf db, dx;
+    int rc;
+    if ((rc = db.alloc(16ull * n * nrhs)) || (rc = dx.alloc(16ull * n * nrhs))) return rc;
+    if (hipMemcpyAsync(db.p, rhs, 16ull * n * nrhs, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (opts->use_x0 && hipMemcpyAsync(dx.p, x, 16ull * n * nrhs, hipMemcpyHostToDevice, s) != hipSuccess))
+        return IFS_E_HIP;
+    rc = ifs_solve_block_device(c, nrhs, db.p, dx.p, n, n, opts, info);
+    if (rc == IFS_OK || rc == IFS_E_NOCONV || rc == IFS_E_BREAKDOWN) {
+        if (hipMemcpyAsync(x, dx.p, 16ull * n * nrhs, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return IFS_E_HIP;
+    }
+    return rc;
 }
 
 extern "C" int ifs_solve_multi(ifs_ctx *c, const double *rhs, double *x, int64_t n, const ifs_solve_opts *opts,

@@ -200,6 +200,20 @@ class MultiSphereSolver:
         return X, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
                         rel_residual_flat=i.rel_residual_flat, seconds=i.seconds) for i in infos]
 
+    def solve_block(self, RHS, restart=30, max_iter=1000, tol=1e-6, raise_on_noconv=True, precision="fp64",
+                    verbose=False):
+        """Block GMRES for the rows of RHS [nrhs, n] (one shared Krylov space); returns (X, [info dicts])."""
+        Bm = np.ascontiguousarray(np.atleast_2d(RHS), dtype=np.complex128)
+        nr = Bm.shape[0]
+        X = np.zeros_like(Bm)
+        op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), 0, int(bool(verbose)), _prec(precision))
+        infos = (_lib.SolveInfo * nr)()
+        rc = lib().ifs_solve_block(self._h, nr, Bm.ctypes.data, X.ctypes.data, self.n, ctypes.byref(op), infos)
+        if rc not in (0, -5, -6) or (rc != 0 and raise_on_noconv):
+            check(rc, "ifs_solve_block")
+        return X, [dict(iterations=i.iterations, converged=bool(i.converged), rel_residual=i.rel_residual,
+                        rel_residual_flat=i.rel_residual_flat, seconds=i.seconds) for i in infos]
+
     def solve_device(self, rhs_ptr, x_ptr, restart=50, max_iter=1000, tol=1e-6, use_x0=False, verbose=False,
                      precision="fp64", graph=False):
         op = _lib.SolveOpts(int(restart), int(max_iter), float(tol), int(bool(use_x0)), int(bool(verbose)),
