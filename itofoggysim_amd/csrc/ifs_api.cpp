@@ -270,7 +270,19 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
         // the separate generate/apply pair spends its time on)
         const bool fused = !B.gq && nrhs == 1 && !B.cached &&
                            ch.max_Lt < (ch.max_nmax + 2 <= 128 ? 256 : 1024) && ch.max_nmax <= 380;  // LDS <= 64 KiB
-        if (!(B.cached && B.tables_ready)) {
+        // sphere <-> box tables not kept: the quadrature is applied to the vectors directly
+        // ((Lt + Ls) Q work per order instead of the table's Lt Ls Q), no table is formed
+        const bool gq_apply = B.gq && !B.cached && gqb_apply_lds(ch.gqb.Q, ch.gqb.Nr) <= 65536;
+        const double *gmu = nullptr, *gw = nullptr;
+        if (B.gq) (void)gq_rule(c, ch.gqb.Q, &gmu, &gw);  // created at finalize
+        if (gq_apply) {
+            {
+                Scope s(c, B.pn[0]);
+                launch_gen_wigner_fold(tt, nt, ch.maxpairs, wd, c->stream);
+            }
+            Scope s(c, B.pn[1]);
+            launch_gqb_legendre(ch.gqb, gmu, c->ws_gq.as<double>(), c->stream);
+        } else if (!(B.cached && B.tables_ready)) {
             {
                 Scope s(c, B.pn[0]);
                 launch_gen_wigner_fold(tt, nt, ch.maxpairs, wd, c->stream);
@@ -303,6 +315,9 @@ int batch_run(ifs_ctx *c, Batch &B, const double2 *src_pool, double2 *dst_pool, 
             if (fused) {
                 Scope s(c, B.pn[4], ch.f_ax);
                 launch_axial_fused(tt, nt, ch.maxK, ch.max_nmax, ch.max_Lt, wsc, wy, wz, c->stream);
+            } else if (gq_apply) {
+                Scope s(c, B.pn[4], ch.f_ax);
+                launch_gqb_apply(tt, nt, ch.gqb, gmu, gw, c->ws_gq.as<double>(), wy, wz, c->stream);
             } else {
                 Scope s(c, B.pn[4], ch.f_ax);
                 launch_axial_apply(tt, nt, ch.max_rows_z, wax, wy, wz, c->stream);

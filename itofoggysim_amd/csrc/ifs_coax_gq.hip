@@ -433,6 +433,126 @@ __global__ void k_gqb_vector(const Task *__restrict__ tasks, int eblk, const dou
     }
 }
 
+// The sphere <-> box coaxial shift applied through the quadrature, without forming
+// its table.  With I(n, nu) = sum_q V_n(mu_q) V_nu(mu_q) T_q (T_q = w_q exp(i t mu_q))
+// and the vector relation of k_gqb_vector, a vector y of source orders shifts as
+//   u(n') = sum_q V_n'(mu_q) T_q g(q),   g(q) = sum_nu V_nu(mu_q) y(nu) / sqrt(nu (nu + 1)),
+//   (P y)(n) = f0 u(n) + i fp u(n+1) - i fm u(n-1) + i fb u(n),   (Q y)(n): -i fb u(n),
+// f0 = sqrt(n(n+1)), fp = t a_n sqrt(n/(n+1)), fm = t a_{n-1} sqrt((n+1)/n) (n-1 >= m),
+// fb = t m / sqrt(n(n+1)).  That is (Lt + Ls) Q multiply-adds per vector and order
+// instead of the table's Lt Ls Q.  Block per (task, m, sign of k): the two vectors
+// y_+(., k), y_-(., k) of order k = +-m; z_+(n, k) = P_m y_+ and z_-(n, k) = Q_m y_- for
+// k = +m, the roles of P and Q exchanged for k = -m.  V: the chunk's table of Pb_n^m at
+// the nodes (k_gq_leg), [m][n][q].  LDS: 2 (Nr + 2) + 2 Q complex.
+__global__ __launch_bounds__(256) void k_gqb_apply(const Task *__restrict__ tasks, int Q, int Nr,
+                                                   const double *__restrict__ mu, const double *__restrict__ w,
+                                                   const double *__restrict__ V, const double2 *__restrict__ wy,
+                                                   double2 *__restrict__ wz) {
+    extern __shared__ double2 shq[];
+    const Task T = tasks[blockIdx.x];
+    const int m = blockIdx.y;
+    const int sgn = blockIdx.z == 0 ? 1 : -1;
+    if (m > T.K || (m == 0 && sgn < 0)) return;
+    const int Lt = T.Lt, Ls = T.Ls, K = T.K;
+    const int k = sgn * m;
+    const int q0 = m < 1 ? 1 : m;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double2 *yv = shq;                 // [2][Nr + 2]: y / sqrt(nu(nu+1)), then u
+    double2 *gq = shq + 2 * (Nr + 2);  // [2][Q]
+    const int64_t ny = rows_size(Ls, K), nz = rows_size(Lt, K);
+    const double2 *ya = wy + T.off_y, *yb = ya + ny;
+    for (int nu = q0 + tid; nu <= Ls; nu += blockDim.x) {
+        const int kk = nu < K ? nu : K;
+        const int64_t iy = rows_off(nu, K) + k + kk;
+        const double inv = 1.0 / sqrt((double)nu * (nu + 1.0));
+        const double2 a = ya[iy], b = yb[iy];
+        yv[nu] = make_double2(inv * a.x, inv * a.y);
+        yv[(Nr + 2) + nu] = make_double2(inv * b.x, inv * b.y);
+    }
+    __syncthreads();
+    const double *Vm = V + (int64_t)m * (Nr + 1) * Q;
+    // g(q) = T_q sum_nu V_nu(q) yv(nu): threads along q (coalesced table rows)
+    for (int q = tid; q < Q; q += blockDim.x) {
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        for (int nu = q0; nu <= Ls; ++nu) {
+            const double v = Vm[(int64_t)nu * Q + q];
+            const double2 a = yv[nu], b = yv[(Nr + 2) + nu];
+            ar = fma(v, a.x, ar);
+            ai = fma(v, a.y, ai);
+            br = fma(v, b.x, br);
+            bi = fma(v, b.y, bi);
+        }
+        double sn, cs;
+        sincos(T.t * mu[q], &sn, &cs);
+        const double tr = w[q] * cs, ti = w[q] * sn;
+        gq[q] = make_double2(tr * ar - ti * ai, tr * ai + ti * ar);
+        gq[Q + q] = make_double2(tr * br - ti * bi, tr * bi + ti * br);
+    }
+    __syncthreads();
+    // u(n') = sum_q V_n'(q) g(q), n' = m .. Lt + 1: a wave per row, lanes along q
+    for (int n = m + wv; n <= Lt + 1; n += (int)(blockDim.x >> 6)) {
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        const double *Vn = Vm + (int64_t)n * Q;
+        for (int q = lane; q < Q; q += 64) {
+            const double v = Vn[q];
+            const double2 a = gq[q], b = gq[Q + q];
+            ar = fma(v, a.x, ar);
+            ai = fma(v, a.y, ai);
+            br = fma(v, b.x, br);
+            bi = fma(v, b.y, bi);
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            ar += __shfl_xor(ar, o, 64);
+            ai += __shfl_xor(ai, o, 64);
+            br += __shfl_xor(br, o, 64);
+            bi += __shfl_xor(bi, o, 64);
+        }
+        if (lane == 0) {
+            yv[n] = make_double2(ar, ai);
+            yv[(Nr + 2) + n] = make_double2(br, bi);
+        }
+    }
+    __syncthreads();
+    // z_+(n, k), z_-(n, k): P_m / Q_m on y_+ / y_- for k = +m, exchanged for k = -m
+    double2 *z = wz + T.off_z;
+    const double t = T.t;
+    for (int n = q0 + tid; n <= Lt; n += blockDim.x) {
+        const double nn = (double)n;
+        const double f0 = sqrt(nn * (nn + 1.0)), fp = t * coef_ag(n, m) * sqrt(nn / (nn + 1.0));
+        const double fm = n - 1 >= m ? t * coef_ag(n - 1, m) * sqrt((nn + 1.0) / nn) : 0.0;
+        const double fb = t * (double)m / sqrt(nn * (nn + 1.0));
+        const int kk = n < K ? n : K;
+        const int64_t e = rows_off(n, K) + k + kk;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const double2 *u = yv + v * (Nr + 2);
+            const double2 u0 = u[n], up = u[n + 1];
+            const double2 um = n - 1 >= m ? u[n - 1] : make_double2(0.0, 0.0);
+            // A part: f0 u(n) + i fp u(n+1) - i fm u(n-1); B part: i fb u(n)
+            const double2 av = make_double2(f0 * u0.x - fp * up.y + fm * um.y, f0 * u0.y + fp * up.x - fm * um.x);
+            const double2 bv = make_double2(-fb * u0.y, fb * u0.x);
+            // P = A + B on y_+ for k = +m and on y_- for k = -m; Q = A - B otherwise
+            const bool plus = (v == 0) == (sgn > 0);
+            z[e + v * nz] = plus ? make_double2(av.x + bv.x, av.y + bv.y) : make_double2(av.x - bv.x, av.y - bv.y);
+        }
+    }
+}
+
+size_t gqb_apply_lds(int Q, int Nr) { return sizeof(double2) * (2 * (size_t)(Nr + 2) + 2 * (size_t)Q); }
+
+void launch_gqb_legendre(const GqbChunk &g, const double *mu, double *V, hipStream_t s) {
+    hipLaunchKernelGGL(k_gq_leg, dim3((g.Q + 255) / 256, g.maxK + 1), dim3(256), 0, s, 0, g.maxK + 1, g.Nr, g.Q, mu,
+                       V);
+}
+
+void launch_gqb_apply(const Task *tasks, int ntasks, const GqbChunk &g, const double *mu, const double *w,
+                      const double *V, const double2 *wy, double2 *wz, hipStream_t s) {
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(k_gqb_apply, dim3((unsigned)ntasks, g.maxK + 1, 2), dim3(256), gqb_apply_lds(g.Q, g.Nr), s,
+                       tasks, g.Q, g.Nr, mu, w, V, wy, wz);
+}
+
 int gqb_nodes(double t, int Lt, int Ls) {
     const int Pdeg = (int)ceil(t + 8.0 * cbrt(t) + 30.0);
     return (Lt + 1 + Ls + Pdeg) / 2 + 4;

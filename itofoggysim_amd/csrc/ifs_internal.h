@@ -162,6 +162,13 @@ struct GqbChunk {
 int gqb_nodes(double t, int Lt, int Ls);
 inline int64_t gqb_alpha_size(int Lt, int Ls, int K) { return (int64_t)(K + 1) * (Lt + 2) * (Ls + 1); }
 // V: (maxK + 1) (Nr + 1) Q doubles of workspace
+// Sphere <-> box coaxial shift applied through the quadrature (no table): the chunk's
+// Legendre table V at the nodes (launch_gqb_legendre), then per right-hand side
+// launch_gqb_apply (Y -> Z of every task).  Dynamic LDS: gqb_apply_lds(Q, Nr) bytes.
+size_t gqb_apply_lds(int Q, int Nr);
+void launch_gqb_legendre(const GqbChunk &g, const double *mu, double *V, hipStream_t s);
+void launch_gqb_apply(const Task *tasks, int ntasks, const GqbChunk &g, const double *mu, const double *w,
+                      const double *V, const double2 *wy, double2 *wz, hipStream_t s);
 void launch_gen_axial_gqb(const Task *tasks, int ntasks, const GqbChunk &g, const double *mu, const double *w,
                           double *V, double2 *walpha, double2 *wax, hipStream_t s);
 
