@@ -906,6 +906,7 @@ double ctx_k(ifs_ctx *c) { return c->prm.k; }
 
 double2 *c_bhat(ifs_ctx *c, int64_t n) { return c->k_bhat.alloc(16ull * n) ? nullptr : c->k_bhat.as<double2>(); }
 double2 *c_xhat(ifs_ctx *c, int64_t n) { return c->k_xhat.alloc(16ull * n) ? nullptr : c->k_xhat.as<double2>(); }
+double2 *c_stage(ifs_ctx *c, int64_t n) { return c->k_stage.alloc(16ull * n) ? nullptr : c->k_stage.as<double2>(); }
 
 int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **w, double2 **r, double2 **part,
                      double2 **h, int nblocks) {
@@ -1216,9 +1217,15 @@ int matvec_graph(ifs_ctx *c, const double2 *x, double2 *b, int normalised, int n
     if (g && g->exec && g->sig == sig)
         return hipGraphLaunch(g->exec, c->stream) == hipSuccess ? IFS_OK : IFS_E_HIP;
     if (!g) {
-        // first call with these arguments: eager (sizes every buffer, builds kept tables)
+        // first call with these arguments: eager (sizes every buffer, builds kept tables);
+        // at most kMaxGraphs argument sets are kept, the oldest dropped first
         int rc = matvec_impl(c, x, b, normalised, nrhs, ld, ldb, prec);
         if (rc) return rc;
+        constexpr size_t kMaxGraphs = 8;
+        if (c->graphs.size() >= kMaxGraphs) {
+            if (c->graphs.front().exec) (void)hipGraphExecDestroy(c->graphs.front().exec);
+            c->graphs.erase(c->graphs.begin());
+        }
         c->graphs.push_back({x, b, normalised, nrhs, prec, ld, ldb, 0, nullptr});
         return IFS_OK;
     }

@@ -152,6 +152,7 @@ struct ifs_ctx {
 
     // Krylov workspace
     ifs::DevBuf k_V, k_w, k_r, k_part, k_h, k_bhat, k_xhat;
+    ifs::DevBuf k_stage;            // graph-mode staging of the Krylov vector (stable address)
     int k_restart = 0;
 
     ifs::Profiler prof;

@@ -34,6 +34,7 @@ int ctx_krylov_alloc(ifs_ctx *c, int restart, int64_t n, double2 **V, double2 **
                      double2 **h, int nblocks);
 double2 *c_bhat(ifs_ctx *c, int64_t n);
 double2 *c_xhat(ifs_ctx *c, int64_t n);
+double2 *c_stage(ifs_ctx *c, int64_t n);
 int ctx_ns(ifs_ctx *c);
 int ctx_N(ifs_ctx *c);
 const double *ctx_pos(ifs_ctx *c);
@@ -98,9 +99,8 @@ extern "C" int ifs_solve_device(ifs_ctx *c, const void *rhs_dev, void *x_dev, in
     // graph mode: the Arnoldi products go through one staging vector, so that one
     // captured matvec serves every Krylov vector
     const bool graph = (opts->flags & IFS_SOLVE_GRAPH) != 0;
-    DevBuf dstage;
-    if (graph && (rc = dstage.alloc(16ull * n))) return rc;
-    double2 *stage = dstage.as<double2>();
+    double2 *stage = graph ? c_stage(c, n) : nullptr;  // context-owned: later solves replay the same graph
+    if (graph && !stage) return IFS_E_NOMEM;
     // r = b - M x
     auto residual = [&]() -> int {
         int q = graph ? matvec_graph(c, x, r, 1, 1, n) : matvec_impl(c, x, r, 1, 1, n);
