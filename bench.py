@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--serve-nmax", type=int, default=500,
                     help="largest cluster of the serving workload (the stated workload goes to 20000)")
     ap.add_argument("--serve-dirs", type=int, default=8, help="most incidence directions of a serving request")
+    ap.add_argument("--serve-verbose", action="store_true",
+                    help="serving progress on stderr (each request's GMRES steps and completion)")
     ap.add_argument("--latency-seeds", type=int, default=2,
                     help="clusters (seeds 2000, 2001, ...) for the end-to-end solve latency; 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -487,6 +489,8 @@ def main():
         reqs = poisson_workload(args.serve_requests, args.serve_rate, seed=5, n_min=10, n_max=args.serve_nmax,
                                 max_dirs=args.serve_dirs)
         srv = Server(slots=8, hbm_budget_gb=200.0)
+        if args.serve_verbose:
+            srv.set_verbose(True)
         t0 = time.perf_counter()
         res, stt = run_workload(srv, reqs, tol=1e-6)
         wall = time.perf_counter() - t0

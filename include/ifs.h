@@ -317,6 +317,8 @@ int ifs_server_wait(ifs_server *srv, int64_t request_id, double *csec, ifs_reque
 int ifs_server_stats(ifs_server *srv, double out[6]);
 /* Finishes the queued requests, then stops the workers. */
 int ifs_server_destroy(ifs_server *srv);
+/* 1: each request's GMRES steps and its completion are reported on stderr; 0 (default): silent. */
+int ifs_server_set_verbose(ifs_server *srv, int32_t verbose);
 
 /* Interaction counts of the MLFMM plan per kind (P2M, M2M, M2L, L2L, L2P, P2P)
  * and the box truncation per level (levels_L[l] for l = 0..31, 0 if unused). */

@@ -106,6 +106,7 @@ SIGNATURES = {
     "ifs_server_wait": (_i32, [_vp, _i64, _vp, ctypes.POINTER(RequestInfo)]),
     "ifs_server_stats": (_i32, [_vp, _vp]),
     "ifs_server_destroy": (_i32, [_vp]),
+    "ifs_server_set_verbose": (_i32, [_vp, _i32]),
     "ifs_tree_build_gpu": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
     "ifs_matvec_graph_device": (_i32, [_vp, _vp, _vp, _i64]),
     "ifs_class_tile_stats": (_i32, [_vp, _vp]),

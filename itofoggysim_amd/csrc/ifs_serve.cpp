@@ -17,6 +17,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <cstdio>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -57,6 +58,7 @@ struct ifs_server {
     int running = 0, peak_running = 0;
     int64_t next_id = 1;
     bool stop = false;
+    int verbose = 0;  // progress lines on stderr: each request's GMRES steps and its completion
     clk::time_point t0;
     std::vector<double> latencies;
     int64_t completed = 0;
@@ -165,7 +167,7 @@ Plan plan_request(Req &r, size_t budget, int slots) {
     return P;
 }
 
-void run_request(Req &r) {
+void run_request(Req &r, int verbose) {
     ifs_cluster cl{(int32_t)r.radii.size(), r.centres.data(), r.radii.data(), r.mrel.data(),
                    r.lmax.empty() ? nullptr : r.lmax.data()};
     ifs_ctx *c = nullptr;
@@ -183,7 +185,7 @@ void run_request(Req &r) {
     }
     std::vector<ifs_solve_info> info(nd);
     if (!rc) {
-        ifs_solve_opts op{r.restart, r.max_iter, r.tol, 0, 0, r.prm.precision, 0};
+        ifs_solve_opts op{r.restart, r.max_iter, r.tol, 0, verbose, r.prm.precision, 0};
         rc = ifs_solve_batch(c, nd, rhs.data(), x.data(), n, &op, info.data());
     }
     r.csec.assign(3 * nd, 0.0);
@@ -230,10 +232,20 @@ void worker(ifs_server *s) {
         s->running += 1;
         s->peak_running = std::max(s->peak_running, s->running);
         r.t_start = clk::now();
+        const int verbose = s->verbose;
         lk.unlock();
-        run_request(r);  // r stays valid: map nodes are stable and erased only by the waiter
+        run_request(r, verbose);  // r stays valid: map nodes are stable and erased only by the waiter
         lk.lock();
         r.t_done = clk::now();
+        if (verbose) {
+            std::fprintf(stderr, "serve: request %lld done: %zu spheres, %zu directions, %s, %.1f s (queued %.1f s), "
+                                 "status %d\n",
+                         (long long)r.id, r.radii.size(), r.alpha.size(),
+                         r.prm.precision == IFS_PREC_MIXED ? "mixed" : "fp64",
+                         std::chrono::duration<double>(r.t_done - r.t_submit).count(),
+                         std::chrono::duration<double>(r.t_start - r.t_submit).count(), r.status);
+            std::fflush(stderr);
+        }
         r.done = true;
         s->admitted -= r.footprint;
         s->running -= 1;
@@ -247,6 +259,13 @@ void worker(ifs_server *s) {
 }  // namespace
 
 extern "C" {
+
+int ifs_server_set_verbose(ifs_server *s, int32_t verbose) {
+    if (!s || (verbose != 0 && verbose != 1)) return IFS_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->verbose = verbose;
+    return IFS_OK;
+}
 
 int ifs_server_create(ifs_server **out, int32_t slots, double hbm_budget_gb) {
     if (!out || slots < 1 || slots > 64 || !(hbm_budget_gb > 0.0)) return IFS_E_ARG;

@@ -400,7 +400,7 @@ int solve_systems(ifs_ctx *c, const Systems &L, const double2 *bin, double2 *xou
                     launch_scale_copy(make_double2(1.0 / hn, 0.0), w, kv(r, j + 1), len, s);
                 }
             }
-            if (opts->verbose && ns > 1) {
+            if (opts->verbose) {
                 int nact = 0;
                 for (int r = 0; r < ns; ++r) nact += !st[r].done;
                 double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

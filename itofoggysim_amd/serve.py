@@ -49,6 +49,10 @@ class Server:
         self._ndir[rid.value] = len(al)
         return rid.value
 
+    def set_verbose(self, verbose=True):
+        """Progress on stderr: each request's GMRES steps and its completion."""
+        check(lib().ifs_server_set_verbose(self._h, int(bool(verbose))), "ifs_server_set_verbose")
+
     def wait(self, rid):
         """(cross sections [ndir, 3], info dict) of a request."""
         nd = self._ndir.pop(rid)
