@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--serve-nmax", type=int, default=500,
                     help="largest cluster of the serving workload (the stated workload goes to 20000)")
     ap.add_argument("--serve-dirs", type=int, default=8, help="most incidence directions of a serving request")
+    ap.add_argument("--serve-slots", type=int, default=4,
+                    help="serving worker slots (requests running at once; 4 = the process's hardware queues)")
     ap.add_argument("--serve-verbose", action="store_true",
                     help="serving progress on stderr (each request's GMRES steps and completion)")
     ap.add_argument("--latency-seeds", type=int, default=2,
@@ -488,7 +490,7 @@ def main():
         torch.cuda.synchronize()
         reqs = poisson_workload(args.serve_requests, args.serve_rate, seed=5, n_min=10, n_max=args.serve_nmax,
                                 max_dirs=args.serve_dirs)
-        srv = Server(slots=8, hbm_budget_gb=200.0)
+        srv = Server(slots=args.serve_slots, hbm_budget_gb=200.0)
         if args.serve_verbose:
             srv.set_verbose(True)
         t0 = time.perf_counter()
@@ -501,7 +503,7 @@ def main():
             "workload": f"{len(reqs)} requests, Poisson arrivals at {args.serve_rate} /s, N log-uniform in "
                         f"[10, {args.serve_nmax}] (20-sphere benchmark radius law and number density), lambda "
                         f"U[400,1550] nm with n = 1.3199 + 3500 nm^2/lambda^2 + 1e-8 i, 1-{args.serve_dirs} directions, "
-                        f"half plain-FP64 / half mixed, GMRES(30) to 1e-6, 8 slots, 200 GB budget, shortest estimated request first with aging",
+                        f"half plain-FP64 / half mixed, GMRES(30) to 1e-6, {args.serve_slots} slots, 200 GB budget, shortest estimated request first with aging",
             "all_ok": ok, "requests": len(reqs), "solves": nsolves, "wall_s": round(wall, 3),
             "p50_latency_s": round(float(stt["p50_s"]), 4), "p99_latency_s": round(float(stt["p99_s"]), 4),
             "requests_per_s": round(len(reqs) / wall, 4), "solves_per_s": round(nsolves / wall, 4),
