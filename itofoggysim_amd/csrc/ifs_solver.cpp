@@ -500,7 +500,6 @@ static int solve_block(ifs_ctx *c, int p, const double2 *bin, double2 *xout, int
     };
     // orthonormalises the p columns at Q (stride n) among themselves (MGS, two passes);
     // T[q' + p q] receives the upper-triangular factor (column-major p x p)
-    std::vector<cplx> one_h(1);
     auto block_qr = [&](double2 *Q, std::vector<cplx> &T) -> int {
         T.assign((size_t)p * p, 0.0);
         std::vector<double> nrm0(p);
@@ -534,7 +533,7 @@ static int solve_block(ifs_ctx *c, int p, const double2 *bin, double2 *xout, int
         }
         return IFS_OK;
     };
-    int iters = 0, cycles = 0;
+    int iters = 0;
     std::vector<int> conv(p, 0);
     bool first = true;
     for (;;) {
@@ -563,7 +562,6 @@ static int solve_block(ifs_ctx *c, int p, const double2 *bin, double2 *xout, int
             std::fflush(stderr);
         }
         if (all || iters >= opts->max_iter) break;
-        ++cycles;
         // V_0 S = R
         if (hipMemcpyAsync(V, R, 16ull * n * p, hipMemcpyDeviceToDevice, s) != hipSuccess) return IFS_E_HIP;
         std::vector<cplx> S;
@@ -690,7 +688,6 @@ static int solve_block(ifs_ctx *c, int p, const double2 *bin, double2 *xout, int
         }
         all = all && conv[q];
     }
-    (void)cycles;
     return all ? IFS_OK : IFS_E_NOCONV;
 }
 
