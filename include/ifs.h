@@ -317,6 +317,12 @@ int ifs_server_wait(ifs_server *srv, int64_t request_id, double *csec, ifs_reque
 int ifs_server_stats(ifs_server *srv, double out[6]);
 /* Finishes the queued requests, then stops the workers. */
 int ifs_server_destroy(ifs_server *srv);
+/* The admission plan of a request, on the host only (no device work): out = bytes of
+ * vectors, MLFMM pools, translation workspace, kept tables, fixed buffers, their total,
+ * then the workspace_mb and cache_mb (-1: none kept) its context would be created with,
+ * for ndir directions, GMRES(restart), a budget of hbm_budget_gb shared by slots. */
+int ifs_server_plan(const ifs_cluster *cluster, const ifs_params *params, int32_t ndir, int32_t restart,
+                    double hbm_budget_gb, int32_t slots, double out[8]);
 /* 1: each request's GMRES steps and its completion are reported on stderr; 0 (default): silent. */
 int ifs_server_set_verbose(ifs_server *srv, int32_t verbose);
 

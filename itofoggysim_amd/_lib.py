@@ -107,6 +107,8 @@ SIGNATURES = {
     "ifs_server_stats": (_i32, [_vp, _vp]),
     "ifs_server_destroy": (_i32, [_vp]),
     "ifs_server_set_verbose": (_i32, [_vp, _i32]),
+    "ifs_server_plan": (_i32, [ctypes.POINTER(Cluster), ctypes.POINTER(Params), _i32, _i32, ctypes.c_double, _i32,
+                               _vp]),
     "ifs_tree_build_gpu": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
     "ifs_matvec_graph_device": (_i32, [_vp, _vp, _vp, _i64]),
     "ifs_class_tile_stats": (_i32, [_vp, _vp]),

@@ -313,6 +313,32 @@ int ifs_server_submit(ifs_server *s, const ifs_cluster *cl, const ifs_params *pr
     return IFS_OK;
 }
 
+int ifs_server_plan(const ifs_cluster *cl, const ifs_params *prm, int32_t ndir, int32_t restart,
+                    double hbm_budget_gb, int32_t slots, double out[8]) {
+    if (!cl || !prm || !out || ndir < 1 || cl->n_spheres < 1 || !cl->centres || !cl->radii || restart < 1 ||
+        slots < 1 || !(hbm_budget_gb > 0.0))
+        return IFS_E_ARG;
+    Req r;
+    const int ns = cl->n_spheres;
+    r.centres.assign(cl->centres, cl->centres + 3 * ns);
+    r.radii.assign(cl->radii, cl->radii + ns);
+    if (cl->lmax) r.lmax.assign(cl->lmax, cl->lmax + ns);
+    r.alpha.assign((size_t)ndir, 0.0);
+    r.beta.assign((size_t)ndir, 0.0);
+    r.prm = *prm;
+    r.restart = restart;
+    const Plan P = plan_request(r, (size_t)(hbm_budget_gb * 1073741824.0), slots);
+    out[0] = (double)P.vectors;
+    out[1] = (double)P.pools;
+    out[2] = (double)P.workspace;
+    out[3] = (double)P.tables;
+    out[4] = (double)P.fixed;
+    out[5] = (double)P.total();
+    out[6] = (double)r.prm.workspace_mb;
+    out[7] = (double)r.prm.cache_mb;
+    return IFS_OK;
+}
+
 int ifs_server_wait(ifs_server *s, int64_t id, double *csec, ifs_request_info *info) {
     if (!s) return IFS_E_ARG;
     std::unique_lock<std::mutex> lk(s->mu);

@@ -82,6 +82,33 @@ class Server:
             pass
 
 
+def plan_request(centres, radii, k, ndir, lmax=None, restart=30, n_global=None, use_fmm=None, leaf_size=32,
+                 fmm_d0=6.0, hbm_budget_gb=200.0, slots=4):
+    """The server's admission plan of one request, on the host (ifs_server_plan): bytes of
+    vectors, MLFMM pools, workspace, kept tables, fixed buffers and their total, and the
+    workspace_mb / cache_mb its context would get (cache_mb -1: no tables kept)."""
+    centres = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    ns = radii.shape[0]
+    if lmax is None:
+        x = k * radii
+        lmax = np.ceil(x + 4.05 * np.cbrt(x) + 2.0).astype(np.int32)
+    lmax = np.ascontiguousarray(lmax, dtype=np.int32)
+    m = np.ones(ns, dtype=np.complex128)
+    cl = _lib.Cluster(ns, centres.ctypes.data, radii.ctypes.data, m.ctypes.data, lmax.ctypes.data)
+    pr = _lib.Params()
+    pr.k = float(k)
+    pr.n_global = int(n_global if n_global is not None else lmax.max())
+    pr.use_fmm = int(ns > 200 if use_fmm is None else bool(use_fmm))
+    pr.leaf_size = int(leaf_size)
+    pr.fmm_d0 = float(fmm_d0)
+    out = np.zeros(8)
+    check(lib().ifs_server_plan(ctypes.byref(cl), ctypes.byref(pr), int(ndir), int(restart), float(hbm_budget_gb),
+                                int(slots), out.ctypes.data), "ifs_server_plan")
+    keys = ["vectors", "pools", "workspace", "tables", "fixed", "total", "workspace_mb", "cache_mb"]
+    return {k_: (int(v) if k_ in ("workspace_mb", "cache_mb") else float(v)) for k_, v in zip(keys, out)}
+
+
 def water_index(wavelength_um):
     """n(lambda) = 1.3199 + 3500 nm^2 / lambda^2, k = 1e-8 (lambda in um)."""
     return complex(1.3199 + 3500e-6 / wavelength_um ** 2, 1e-8)
