@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 METRIC = "MLFMM matvecs/s & solves/s at rel. residual 1e-6; p50/p99 solve latency"
 
 
@@ -440,7 +441,14 @@ def main():
         "kernels_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) for k, v in kern.items() if v[0] > 0},
         "finite": finite,
     }
+    kern32 = {k: v for k, v in prof32.items() if v[2] > 0 and v[0] > 0}
+    dom32 = max(kern32, key=lambda k: kern32[k][0])
+    a32 = kern32[dom32][2] / (kern32[dom32][0] * 1e-3) / 1e12
+    peak32 = FP32_PEAK_TFLOPS if dom32.endswith("_f32") else FP64_PEAK_TFLOPS
     out["mixed"] = {
+        "roofline": {"bound": "mfma", "kernel": dom32, "achieved": round(a32, 3), "peak": peak32,
+                     "unit": "TFLOP/s", "frac": round(a32 / peak32, 4), "launches": kern32[dom32][1],
+                     "avg_launch_ms": round(kern32[dom32][0] / kern32[dom32][1], 4)},
         "value": round(1000.0 / ms32, 6), "unit": "matvecs/s", "ms_per_step": round(ms32, 3),
         "speedup_vs_fp64": round(ms / ms32, 3),
         "rel_error_vs_fp64_normalised": mixed_err,
