@@ -26,7 +26,7 @@ def _far_field_grid(s, x):
     return np.concatenate([ft, fp])
 
 
-def test_config2_full_size(oracle):
+def test_config2_full_size(oracle, progress):
     c = config_cluster(2)
     lm = c["lmax"]
     # --- MLFMM: matvec, FP64 and mixed solves
@@ -38,10 +38,13 @@ def test_config2_full_size(oracle):
     xh = (rng.standard_normal(f.n) + 1j * rng.standard_normal(f.n)) * mask
     x = xh / S
     bf = f.calculate_vector_b(x) * S
+    progress("MLFMM context built, first matvec done (kept tables generated)")
     rhs = f.get_vector_b()
     p = f.plane_wave(0.0, 0.0)
     xf64, if64 = f.solve(rhs, restart=50, tol=1e-6, x0=rhs)
+    progress(f"MLFMM FP64 GMRES(50): {if64['iterations']} products")
     xmix, imix = f.solve(rhs, restart=50, tol=1e-6, x0=rhs, precision="mixed")
+    progress(f"MLFMM mixed GMRES(50): {imix['iterations']} products")
     print("MLFMM FP64 solve", if64, "\nMLFMM mixed solve", imix)
     assert if64["converged"] and imix["converged"]
     assert if64["rel_residual"] < 1e-6 and imix["rel_residual"] < 1e-6
@@ -50,9 +53,11 @@ def test_config2_full_size(oracle):
     ff_f64 = _far_field_grid(f, xf64)
     ff_mix = _far_field_grid(f, xmix)
     f.close()
+    progress("MLFMM cross sections and far fields done")
     # --- direct: matvec, one target's rows against the oracle, FP64 solve
     d = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=lm)
     bd = d.calculate_vector_b(x) * S
+    progress("direct context built, first matvec done")
     err_fmm = np.linalg.norm(bf - bd) / np.linalg.norm(bd - xh)
     print(f"MLFMM vs direct matvec (normalised basis): {err_fmm:.2e}")
     assert err_fmm < 1e-8
@@ -61,9 +66,11 @@ def test_config2_full_size(oracle):
     src = [i for i in range(len(lm)) if i != j]
     ref = (x + oracle.matvec_pairs(-c["centres"], lm, N, an, bn, c["k"], x, [j] * len(src), src)) * S
     e_or = per_sphere_rel(bd, ref, xh, len(lm), N)[j]
+    progress(f"oracle rows of sphere {j} over {len(src)} sources done")
     print(f"direct rows of sphere {j} (L = {lm[j]}) vs oracle over {len(src)} sources: {e_or:.2e}")
     assert e_or < 1e-10
     xd, idd = d.solve(rhs, restart=50, tol=1e-6, x0=rhs)
+    progress(f"direct FP64 GMRES(50): {idd['iterations']} products")
     print("direct FP64 solve", idd)
     assert idd["converged"]
     cs_d = np.array(d.cross_sections(xd, p))

@@ -25,7 +25,7 @@ from itofoggysim_amd import MultiSphereSolver, config_cluster
 pytestmark = pytest.mark.gpu
 
 
-def test_config3_1550nm(oracle):
+def test_config3_1550nm(oracle, progress):
     c = config_cluster(3, wavelength=1.55)
     lm = c["lmax"]
     ns = len(lm)
@@ -54,12 +54,15 @@ def test_config3_1550nm(oracle):
         s.set_precision("mixed")
         b32 = s.calculate_vector_b(x) * S
         s.set_precision("fp64")
+        progress("context built; FP64 and mixed matvecs done (oracle rows running on the host)")
         emix = np.linalg.norm(b32 - b64) / np.linalg.norm(b64 - xh)
         print(f"mixed vs FP64 matvec (normalised basis): {emix:.2e}", flush=True)
         rhs = s.get_vector_b()
         x64, i64 = s.solve(rhs, restart=30, tol=1e-6, x0=rhs, max_iter=400)
+        progress(f"FP64 GMRES(30): {i64['iterations']} products")
         print("FP64 GMRES(30)", i64, flush=True)
         x32, i32 = s.solve(rhs, restart=30, tol=1e-6, x0=rhs, max_iter=400, precision="mixed")
+        progress(f"mixed GMRES(30): {i32['iterations']} products")
         print("mixed GMRES(30)", i32, flush=True)
         p = s.plane_wave(0.0, 0.0)
         cs64 = np.array(s.cross_sections(x64, p))
@@ -72,7 +75,9 @@ def test_config3_1550nm(oracle):
         s = MultiSphereSolver(c["centres"], c["radii"], c["m_rel"], c["k"], lmax=lm, use_fmm=True, leaf_size=32,
                               fmm_d0=10.0)
         b10 = s.calculate_vector_b(x) * S
+        progress("d0 = 10 context built and matvec done; waiting for the oracle rows")
         th.join()
+        progress("oracle rows done")
     finally:
         s.close()
     for j in targets:
