@@ -344,6 +344,17 @@ int ifs_profile_read(ifs_ctx *ctx, const char **names, double *total_ms, int64_t
  * and the identity. */
 int ifs_matvec_cost(ifs_ctx *ctx, double *flops, double *bytes);
 
+/* Test support, no counterpart in the reference: order k of the coaxial translation
+ * table (kind 1: regular j_n; 3: outgoing -> regular h1_n) of length t = k|d| along
+ * the z axis, truncations Lt (target) and Ls (source), orders |m| <= K, generated on
+ * the device by method 0 (the degree recurrence of the sphere-level tables) or 1 (the
+ * Gauss-Legendre quadrature of the box tables), in the normalised basis.  out receives
+ * 2 (Lt-q+1)(Ls-q+1) complex, q = max(1,|k|): the block P = A + B, then Q = A - B,
+ * each row-major over n = q..Lt (rows) and nu = q..Ls (columns).  Synchronous; it
+ * allocates and frees its own device memory. */
+int ifs_coaxial_block(int32_t kind, double t, int32_t Lt, int32_t Ls, int32_t K, int32_t method, int32_t k,
+                      double *out);
+
 /* Instrumentation: coaxial-shift launches of the box translations since creation,
  * counts[0] with 64-wide translation tiles and counts[1] with the 32-wide tile that a
  * segment remainder of at most 32 translations (columns x right-hand sides) takes. */

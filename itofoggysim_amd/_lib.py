@@ -86,6 +86,7 @@ SIGNATURES = {
     "ifs_profile_enable": (_i32, [_vp, _i32]),
     "ifs_profile_read": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32]),
     "ifs_matvec_cost": (_i32, [_vp, _vp, _vp]),
+    "ifs_coaxial_block": (_i32, [_i32, ctypes.c_double, _i32, _i32, _i32, _i32, _i32, _vp]),
     "ifs_plane_wave": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64]),
     "ifs_plane_wave_rhs_dir": (_i32, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64]),
     "ifs_cross_sections": (_i32, [_vp, _vp, _vp, _i64, _vp]),

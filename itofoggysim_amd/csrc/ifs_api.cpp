@@ -1389,4 +1389,53 @@ int ifs_matvec_cost(ifs_ctx *c, double *flops, double *bytes) {
     return IFS_OK;
 }
 
+int ifs_coaxial_block(int32_t kind, double t, int32_t Lt, int32_t Ls, int32_t K, int32_t method, int32_t k,
+                      double *out) {
+    if ((kind != 1 && kind != 3) || !(t > 0.0) || Lt < 1 || Ls < 1 || K < 0 || K > std::min(Lt, Ls) || k < -K ||
+        k > K || (method != 0 && method != 1) || !out)
+        return IFS_E_ARG;
+    struct Stream {
+        hipStream_t s = nullptr;
+        ~Stream() {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } st;
+    if (hipStreamCreate(&st.s) != hipSuccess) return IFS_E_HIP;
+    DevBuf wax, ws, rule, dtask;
+    int rc;
+    if ((rc = wax.alloc(16 * (size_t)ax_size(Lt, Ls, K)))) return rc;
+    if (method == 0) {
+        // the recurrence generator of the sphere-level tables: sectorial columns, then
+        // the three-term recurrence in the degrees
+        Task T{};
+        T.t = t;
+        T.Lt = Lt;
+        T.Ls = Ls;
+        T.K = K;
+        T.kind = kind;
+        const int nmax = Lt + Ls + 2;
+        if ((rc = dtask.alloc(sizeof(Task))) || (rc = ws.alloc(16 * (size_t)sect_size(K, nmax)))) return rc;
+        if (hipMemcpy(dtask.p, &T, sizeof(Task), hipMemcpyHostToDevice) != hipSuccess) return IFS_E_HIP;
+        launch_gen_sect(dtask.as<Task>(), 1, nmax, ws.as<double2>(), st.s);
+        launch_gen_axial(dtask.as<Task>(), 1, K, nmax, ws.as<double2>(), wax.as<double2>(), st.s);
+    } else {
+        // the Gauss-Legendre generator of the box tables
+        GqPlan G = gq_plan(kind, t, Lt, Ls, K, (size_t)1 << 30);
+        std::vector<double> x, w;
+        gauss_legendre(G.Q, x, w);
+        if ((rc = rule.alloc(16 * (size_t)G.Q)) || (rc = ws.alloc(G.ws_bytes))) return rc;
+        if (hipMemcpy(rule.p, x.data(), 8 * (size_t)G.Q, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(rule.as<double>() + G.Q, w.data(), 8 * (size_t)G.Q, hipMemcpyHostToDevice) != hipSuccess)
+            return IFS_E_HIP;
+        launch_gen_axial_gq(G, rule.as<double>(), rule.as<double>() + G.Q, ws.p, wax.as<double2>(), st.s);
+    }
+    if (hipPeekAtLastError() != hipSuccess) return IFS_E_HIP;
+    const size_t nb = 16 * 2 * (size_t)ax_block(k, Lt, Ls);  // the P block, then the Q block
+    if (hipMemcpyAsync(out, wax.as<double2>() + ax_off(k, Lt, Ls, K), nb, hipMemcpyDeviceToHost, st.s) !=
+            hipSuccess ||
+        hipStreamSynchronize(st.s) != hipSuccess)
+        return IFS_E_HIP;
+    return IFS_OK;
+}
+
 }  // extern "C"
